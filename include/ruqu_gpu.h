@@ -1,0 +1,200 @@
+/*
+ * ruqu_gpu.h — C ABI of the MI355X state-vector engine (libruqu_gpu.so).
+ *
+ * Drop-in boundary for ruqu-core's circuit/state API.  Each entry point names
+ * the Rust item it replaces (paths relative to the ruvnet/ruqu checkout).
+ * Plain pointers and sizes only; every function returns an rq_status and never
+ * aborts.  rq_last_error() gives the message of the last failure on the
+ * calling thread.
+ *
+ * Conventions (identical to the reference):
+ *   - qubit q is bit q of the amplitude index (little-endian; state.rs:128-136)
+ *   - amplitudes are interleaved (re, im) pairs (types.rs:8-11)
+ *   - 2q matrix row/column index = q1_bit*2 + q2_bit (gate.rs:150-153)
+ *   - uniform draws in [0,1) are inputs: the reference draws them from its
+ *     own StdRng (state.rs:302); measurement here consumes caller draws.
+ *
+ * Threading: a state handle owns one HIP stream and must be used by one
+ * thread at a time; distinct handles may be used from distinct threads.
+ */
+#ifndef RUQU_GPU_H
+#define RUQU_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* types.rs:8-11 (f64) and mixed_precision.rs:33-39 (f32). */
+typedef struct { double re, im; } rq_c64;
+typedef struct { float re, im; } rq_c32;
+
+/* error.rs:8-23 variants in declaration order, then the engine's own. */
+typedef enum {
+  RQ_OK = 0,
+  RQ_ERR_QUBIT_LIMIT = 1,     /* QuantumError::QubitLimitExceeded */
+  RQ_ERR_INVALID_QUBIT = 2,   /* QuantumError::InvalidQubitIndex */
+  RQ_ERR_ALLOC = 3,           /* QuantumError::MemoryAllocationFailed (hipMalloc failed) */
+  RQ_ERR_INVALID_STATE = 4,   /* QuantumError::InvalidStateVector */
+  RQ_ERR_CIRCUIT = 5,         /* QuantumError::CircuitError */
+  RQ_ERR_DEVICE = 6,          /* HIP runtime failure or no usable gfx950 device */
+  RQ_ERR_ARGUMENT = 7         /* null pointer / bad size passed across the ABI */
+} rq_status;
+
+/* Precision of a state: complex-f64 (QuantumState) or complex-f32
+ * (QuantumStateF32, mixed_precision.rs:193-200). */
+typedef enum { RQ_C64 = 0, RQ_C32 = 1 } rq_precision;
+
+/* gate.rs:8-36, declaration order. */
+typedef enum {
+  RQ_GATE_H = 0, RQ_GATE_X, RQ_GATE_Y, RQ_GATE_Z, RQ_GATE_S, RQ_GATE_SDG,
+  RQ_GATE_T, RQ_GATE_TDG, RQ_GATE_RX, RQ_GATE_RY, RQ_GATE_RZ, RQ_GATE_PHASE,
+  RQ_GATE_CNOT, RQ_GATE_CZ, RQ_GATE_SWAP, RQ_GATE_RZZ,
+  RQ_GATE_MEASURE, RQ_GATE_RESET, RQ_GATE_BARRIER, RQ_GATE_UNITARY1Q
+} rq_gate_kind;
+
+/* One `Gate` value (gate.rs:8-36): q0 is the (first) qubit, q1 the second
+ * qubit of a 2q gate (CNOT: control=q0, target=q1), theta the angle of
+ * Rx/Ry/Rz/Phase/Rzz, u the matrix of Unitary1Q. 88 bytes. */
+typedef struct {
+  int32_t kind;
+  uint32_t q0, q1, _pad;
+  double theta;
+  rq_c64 u[2][2];
+} rq_gate;
+
+/* types.rs:230-234 MeasurementOutcome. */
+typedef struct {
+  uint32_t qubit;
+  int32_t result;
+  double probability;
+} rq_measurement;
+
+/* types.rs:238-245 SimulationMetrics, plus device-side figures. */
+typedef struct {
+  uint32_t num_qubits;
+  uint32_t _pad;
+  uint64_t gate_count;         /* unitary gates of the input circuit */
+  uint64_t execution_time_ns;  /* host wall clock, like simulator.rs:60,82 */
+  uint64_t peak_memory_bytes;  /* QuantumState::estimate_memory */
+  double gates_per_second;
+  uint64_t gates_fused;        /* input gates folded into fused passes */
+  uint64_t passes;             /* full-state kernel passes launched */
+  double device_time_ms;       /* HIP-event time of the device work */
+  double hbm_bytes;            /* algorithmic HBM bytes of the passes */
+} rq_metrics;
+
+/* types.rs:190-195 PauliOp and one (QubitIndex, PauliOp) factor of a
+ * PauliString (types.rs:201-203). */
+typedef enum { RQ_PAULI_I = 0, RQ_PAULI_X = 1, RQ_PAULI_Y = 2, RQ_PAULI_Z = 3 } rq_pauli_op;
+typedef struct {
+  uint32_t qubit;
+  int32_t op;
+} rq_pauli_factor;
+
+typedef struct rq_state rq_state; /* opaque: owns the device amplitudes */
+
+/* ---- errors / device -------------------------------------------------- */
+const char* rq_last_error(void);
+/* Number of qubits the device can hold for a precision (288 GB HBM3E:
+ * 34 for c64, 35 for c32); the reference's fixed caps are MAX_QUBITS=32
+ * (state.rs:15) and MAX_QUBITS_F32=33 (mixed_precision.rs:185). */
+uint32_t rq_max_qubits(int precision);
+int rq_device_synchronize(void);
+
+/* ---- construction (state.rs:31-96, mixed_precision.rs:206-283) -------- */
+/* QuantumState::new (with_seed=0) / new_with_seed (with_seed=1): |0..0>.
+ * The seed itself is not needed (draws are inputs); with_seed only selects the
+ * reference's validation (new(0) errors, new_with_seed(0) does not, f64). */
+int rq_state_create(uint32_t num_qubits, int precision, int with_seed, rq_state** out);
+/* QuantumState::from_amplitudes: len must equal 2^num_qubits. host_amps holds
+ * rq_c64 (RQ_C64) or rq_c32 (RQ_C32) values. */
+int rq_state_from_amplitudes(const void* host_amps, size_t len, uint32_t num_qubits, int precision,
+                             rq_state** out);
+void rq_state_destroy(rq_state* s);
+uint32_t rq_state_num_qubits(const rq_state* s);      /* state.rs:102 */
+int rq_state_precision(const rq_state* s);
+size_t rq_estimate_memory(uint32_t num_qubits, int precision); /* state.rs:144-146 */
+/* state_vector() (state.rs:110): device -> host copy of all amplitudes. */
+int rq_state_read(const rq_state* s, void* host_amps);
+/* amplitudes_mut() (state.rs:118) write-back: host -> device copy. */
+int rq_state_write(rq_state* s, const void* host_amps);
+/* Reset to |0..0> and clear the measurement record. */
+int rq_state_set_zero(rq_state* s);
+/* dst <- src (same size and precision). */
+int rq_state_copy(const rq_state* src, rq_state* dst);
+/* QuantumStateF32::precision_error_bound (mixed_precision.rs:325-327);
+ * 0 for c64 states. */
+double rq_precision_error_bound(const rq_state* s);
+
+/* ---- gates (state.rs:158-276) ------------------------------------------ */
+/* QuantumState::apply_gate.  Measure/Reset consume `draw`; when the gate is
+ * a Measure, *out receives the outcome and *has_out is set to 1. */
+int rq_apply_gate(rq_state* s, const rq_gate* g, double draw, rq_measurement* out, int* has_out);
+/* apply_single_qubit_gate (state.rs:214-229); m is row-major [2][2]. */
+int rq_apply_1q(rq_state* s, uint32_t qubit, const rq_c64* m);
+/* apply_two_qubit_gate (state.rs:238-276); m is row-major [4][4]. */
+int rq_apply_2q(rq_state* s, uint32_t q1, uint32_t q2, const rq_c64* m);
+
+/* ---- circuits (simulator.rs:56-101) ------------------------------------ */
+/* Simulator::run_with_config on an existing state: the state is first set to
+ * |0..0>, then the gates run in order.  Measure/Reset gates consume draws in
+ * order; measurements receive up to *nout_cap outcomes, *nout the count.
+ * fusion=0: one exact kernel per gate (the reference's arithmetic, without
+ * FMA contraction).  fusion=1: the fused schedule (gates folded into
+ * LDS-resident tiles / dense blocks; rounding differs at the 1e-15 level). */
+int rq_run_circuit(rq_state* s, const rq_gate* gates, size_t ngates, const double* draws, size_t ndraws,
+                   rq_measurement* measurements, size_t nout_cap, size_t* nout, int fusion,
+                   rq_metrics* metrics);
+/* As rq_run_circuit but continue from the current state (no |0..0> reset). */
+int rq_apply_circuit(rq_state* s, const rq_gate* gates, size_t ngates, const double* draws, size_t ndraws,
+                     rq_measurement* measurements, size_t nout_cap, size_t* nout, int fusion,
+                     rq_metrics* metrics);
+/* circuit.rs:153-184 QuantumCircuit::depth. */
+uint32_t rq_circuit_depth(uint32_t num_qubits, const rq_gate* gates, size_t ngates);
+
+/* ---- measurement (state.rs:288-352) ------------------------------------ */
+int rq_measure(rq_state* s, uint32_t qubit, double draw, rq_measurement* out);
+/* state.rs:329-335: draws[q] is consumed by qubit q, qubit 0 first. */
+int rq_measure_all(rq_state* s, const double* draws, rq_measurement* out);
+int rq_reset_qubit(rq_state* s, uint32_t qubit, double draw);
+/* Number of recorded measurements (state.rs:139-141) and a copy of them. */
+size_t rq_measurement_record(const rq_state* s, rq_measurement* out, size_t cap);
+/* Shots without re-simulation: each shot is measure_all (state.rs:329-335)
+ * on a copy of the current state, i.e. Simulator::run_shots
+ * (simulator.rs:107-164) for a circuit without Measure/Reset gates.
+ * draws[shot*n + q] is the draw of qubit q; bits[shot*n + q] receives the
+ * outcome.  *near_ties (optional) counts decisions whose draw lay within
+ * 1e-12 of the threshold (where rounding order could flip the outcome).
+ * The state itself is not collapsed. */
+int rq_sample_shots(const rq_state* s, const double* draws, size_t shots, uint8_t* bits, uint64_t* near_ties);
+
+/* ---- observables (state.rs:123-137, 362-435) --------------------------- */
+int rq_probabilities(const rq_state* s, double* out);
+int rq_probability_of_qubit(const rq_state* s, uint32_t qubit, double* out);
+/* expectation_value of one PauliString (ops in order, duplicates allowed). */
+int rq_expectation(const rq_state* s, const rq_pauli_factor* ops, size_t nops, double* out);
+/* expectation_hamiltonian: term k = coeffs[k] * ops[offsets[k] .. offsets[k+1]). */
+int rq_expectation_hamiltonian(const rq_state* s, const double* coeffs, const uint64_t* offsets,
+                               const rq_pauli_factor* ops, size_t nterms, double* out);
+/* <Z_q> for every qubit in one pass (out has num_qubits entries). */
+int rq_expectation_z_all(const rq_state* s, double* out);
+int rq_normalize(rq_state* s);                                  /* state.rs:415-423 */
+int rq_fidelity(const rq_state* a, const rq_state* b, double* out); /* state.rs:426-435 */
+int rq_norm_sq(const rq_state* s, double* out);
+
+/* ---- timing helpers for the measurement harness ------------------------ */
+/* Time `reps` back-to-back launches of the exact 1q kernel on `qubit` with
+ * matrix m (HIP events on the state's stream); *avg_ms = mean per launch. */
+int rq_time_1q(rq_state* s, uint32_t qubit, const rq_c64* m, int reps, double* avg_ms);
+/* Time `reps` launches of a plain device copy of the state's bytes (read N,
+ * write N) as the achievable-bandwidth reference; *avg_ms per launch. */
+int rq_time_copy(rq_state* s, int reps, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RUQU_GPU_H */

@@ -1,0 +1,336 @@
+// Host planner of the fused schedule (see fusion.h) and the fused runner.
+//
+// Planning steps for one segment (a run of unitary gates between
+// measurements):
+//  1. 1q fusion: consecutive single-qubit gates on a qubit are multiplied
+//     into one 2x2 (C = next * fused, the order of optimizer.rs:70), across
+//     gates on other qubits; a 2q gate on the qubit flushes it.
+//  2. 2q gates become items: CNOT(c,t) = X on t controlled by c; CZ and Rzz =
+//     2-bit diagonals; SWAP = dense 4x4.
+//  3. Stages: greedy scan; an item joins the current stage when none of its
+//     qubits is blocked by an earlier deferred item and its non-diagonal
+//     targets fit in the m-qubit tile set; otherwise it is deferred and
+//     blocks its qubits.  Items on disjoint qubits commute, so the order on
+//     every qubit is preserved.
+//  4. Phases inside a stage: the same greedy rule with 4 register qubits.
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "fusion.h"
+#include "gates.h"
+
+namespace rq {
+
+namespace {
+
+struct Item {
+  int kind;       // FOP_*
+  int q0, q1;     // targets (U1: q0; U2: q0 = row-high, q1; D1: q0; D2: q0 high bit, q1 low bit)
+  int ctl;        // control qubit of a U1 (-1: none)
+  int mat;        // pool offset
+  uint64_t touched() const {
+    uint64_t m = 1ull << q0;
+    if (kind == FOP_U2 || kind == FOP_D2) m |= 1ull << q1;
+    if (ctl >= 0) m |= 1ull << ctl;
+    return m;
+  }
+  uint64_t need() const {  // qubits that must be register qubits
+    if (kind == FOP_U1) return 1ull << q0;
+    if (kind == FOP_U2) return (1ull << q0) | (1ull << q1);
+    return 0;
+  }
+};
+
+inline void mat_mul_2x2(const rq_c64 a[4], const rq_c64 b[4], rq_c64 c[4]) {  // optimizer.rs:12-23
+  c[0] = cadd(cmul(a[0], b[0]), cmul(a[1], b[2]));
+  c[1] = cadd(cmul(a[0], b[1]), cmul(a[1], b[3]));
+  c[2] = cadd(cmul(a[2], b[0]), cmul(a[3], b[2]));
+  c[3] = cadd(cmul(a[2], b[1]), cmul(a[3], b[3]));
+}
+
+int popc(uint64_t x) { return __builtin_popcountll(x); }
+
+}  // namespace
+
+FusionConfig& fusion_config() {
+  static FusionConfig cfg;
+  return cfg;
+}
+
+int plan_fused(uint32_t nq, const rq_gate* gates, size_t ngates, const FusionConfig& cfg, FusedPlan* plan) {
+  plan->stages.clear();
+  plan->phases.clear();
+  plan->ops.clear();
+  plan->pool.clear();
+  plan->unitary = 0;
+  if (nq < (uint32_t)FUSE_R_BITS) return fail(RQ_ERR_ARGUMENT, "fused schedule needs at least 4 qubits");
+  std::vector<rq_c64>& pool = plan->pool;
+
+  // ---- 1. + 2.: items -------------------------------------------------------
+  std::vector<Item> items;
+  std::vector<int> pending(nq, -1);  // pool offset of the pending 2x2 per qubit
+  auto flush = [&](uint32_t q) {
+    if (pending[q] >= 0) {
+      items.push_back(Item{FOP_U1, (int)q, -1, -1, pending[q]});
+      pending[q] = -1;
+    }
+  };
+  for (size_t k = 0; k < ngates; ++k) {
+    const rq_gate& g = gates[k];
+    if (g.kind == RQ_GATE_BARRIER) continue;
+    if (g.kind == RQ_GATE_MEASURE || g.kind == RQ_GATE_RESET)
+      return fail(RQ_ERR_ARGUMENT, "plan_fused: segment contains a measurement");
+    if (g.q0 >= nq || (is_2q(g.kind) && g.q1 >= nq))
+      return fail(RQ_ERR_INVALID_QUBIT, "invalid qubit index " + std::to_string(std::max(g.q0, is_2q(g.kind) ? g.q1 : 0)) +
+                                            " for " + std::to_string(nq) + "-qubit system");
+    ++plan->unitary;
+    if (is_1q(g.kind)) {
+      rq_c64 m[4];
+      matrix_1q(g, m);
+      if (pending[g.q0] >= 0) {
+        rq_c64 c[4];
+        mat_mul_2x2(m, &pool[pending[g.q0]], c);
+        std::memcpy(&pool[pending[g.q0]], c, sizeof(c));
+      } else {
+        pending[g.q0] = (int)pool.size();
+        pool.insert(pool.end(), m, m + 4);
+      }
+      continue;
+    }
+    if (g.q0 == g.q1)
+      return fail(RQ_ERR_CIRCUIT, "two-qubit gate requires distinct qubits, got " + std::to_string(g.q0) + " and " +
+                                      std::to_string(g.q1));
+    flush(g.q0);
+    flush(g.q1);
+    rq_c64 m[16];
+    matrix_2q(g, m);
+    int off = (int)pool.size();
+    switch (g.kind) {
+      case RQ_GATE_CNOT: {
+        rq_c64 x[4] = {{0, 0}, {1, 0}, {1, 0}, {0, 0}};
+        pool.insert(pool.end(), x, x + 4);
+        items.push_back(Item{FOP_U1, (int)g.q1, -1, (int)g.q0, off});
+        break;
+      }
+      case RQ_GATE_CZ:
+      case RQ_GATE_RZZ: {
+        rq_c64 d[4] = {m[0], m[5], m[10], m[15]};
+        pool.insert(pool.end(), d, d + 4);
+        items.push_back(Item{FOP_D2, (int)g.q0, (int)g.q1, -1, off});
+        break;
+      }
+      default:
+        pool.insert(pool.end(), m, m + 16);
+        items.push_back(Item{FOP_U2, (int)g.q0, (int)g.q1, -1, off});
+    }
+  }
+  for (uint32_t q = 0; q < nq; ++q) flush(q);
+  if (items.empty()) return RQ_OK;
+
+  // ---- 3.: stages -----------------------------------------------------------
+  const int m = std::min<int>(std::min(cfg.m, FUSE_MAX_M), (int)nq);
+  const int c = std::max(0, std::min(cfg.c, m - 2));  // leaves room for any 2q target pair
+  if ((int)nq - m > FUSE_MAX_OUTER) return fail(RQ_ERR_ARGUMENT, "state too large for the fused schedule");
+  const uint64_t low = c >= 64 ? ~0ull : ((1ull << c) - 1);
+  std::vector<int> remaining(items.size());
+  for (size_t k = 0; k < items.size(); ++k) remaining[k] = (int)k;
+
+  while (!remaining.empty()) {
+    uint64_t S = low, blocked = 0;
+    std::vector<int> ops, deferred;
+    for (int id : remaining) {
+      const Item& it = items[id];
+      uint64_t t = it.touched();
+      if (t & blocked) {
+        deferred.push_back(id);
+        blocked |= t;
+        continue;
+      }
+      uint64_t ns = S | it.need();
+      if (popc(ns) <= m) {
+        S = ns;
+        ops.push_back(id);
+      } else {
+        deferred.push_back(id);
+        blocked |= t;
+      }
+    }
+    // Fill the tile up to m qubits: first the qubits the deferred items need
+    // (no cost now, and the next stage may start from a similar set), then the
+    // lowest free qubits.
+    for (int id : deferred) {
+      if (popc(S) >= m) break;
+      uint64_t nd = items[id].need() & ~S;
+      while (nd && popc(S) < m) {
+        uint64_t b = nd & (~nd + 1);
+        S |= b;
+        nd &= nd - 1;
+      }
+    }
+    for (uint32_t q = 0; q < nq && popc(S) < m; ++q) S |= 1ull << q;
+
+    FStage st{};
+    st.m = m;
+    int tile_of[64];
+    for (int q = 0; q < 64; ++q) tile_of[q] = -1;
+    int ti = 0, oi = 0;
+    for (uint32_t q = 0; q < nq; ++q) {
+      if (S >> q & 1) {
+        st.S[ti] = (int)q;
+        tile_of[q] = ti++;
+      } else {
+        st.outer[oi++] = (int)q;
+      }
+    }
+    st.nouter = oi;
+    st.phase0 = (int)plan->phases.size();
+
+    // ---- 4.: phases ---------------------------------------------------------
+    std::vector<int> prem = ops;
+    while (!prem.empty()) {
+      uint64_t R = 0, pblocked = 0;
+      std::vector<int> pops, pdef;
+      for (int id : prem) {
+        const Item& it = items[id];
+        uint64_t t = it.touched();
+        if (t & pblocked) {
+          pdef.push_back(id);
+          pblocked |= t;
+          continue;
+        }
+        uint64_t nr = R | it.need();
+        if (popc(nr) <= FUSE_R_BITS) {
+          R = nr;
+          pops.push_back(id);
+        } else {
+          pdef.push_back(id);
+          pblocked |= t;
+        }
+      }
+      // Pad the register set to 4 qubits, preferring tile bits >= 6 so that
+      // the lanes of a wave stay on tile bits 0..5 (direct HBM access).
+      for (int tb = m - 1; tb >= 0 && popc(R) < FUSE_R_BITS; --tb)
+        if (tb >= 6 || m - FUSE_R_BITS < 6) R |= 1ull << st.S[tb];
+      for (int tb = 0; tb < m && popc(R) < FUSE_R_BITS; ++tb) R |= 1ull << st.S[tb];
+
+      FPhase ph{};
+      int nb = 0;
+      int reg_of[64];
+      for (int q = 0; q < 64; ++q) reg_of[q] = -1;
+      for (int tb = 0; tb < m; ++tb)
+        if (R >> st.S[tb] & 1) {
+          reg_of[st.S[tb]] = nb;
+          ph.regt[nb++] = tb;
+        }
+      int thr_of[64];
+      for (int q = 0; q < 64; ++q) thr_of[q] = -1;
+      int nt = 0;
+      for (int tb = 0; tb < m; ++tb)
+        if (!(R >> st.S[tb] & 1)) {
+          thr_of[st.S[tb]] = nt;
+          ph.thrt[nt++] = tb;
+        }
+      bool direct = (m - FUSE_R_BITS) >= 6;
+      for (int b = 0; b < FUSE_R_BITS; ++b) direct = direct && ph.regt[b] >= 6;
+      ph.direct = direct ? 1 : 0;
+      ph.op_begin = (int)plan->ops.size();
+      auto locate = [&](int q, int8_t* loc, int32_t* pos) {
+        if (reg_of[q] >= 0) { *loc = LOC_REG; *pos = reg_of[q]; }
+        else if (thr_of[q] >= 0) { *loc = LOC_THR; *pos = thr_of[q]; }
+        else { *loc = LOC_OUT; *pos = q; }
+      };
+      for (int id : pops) {
+        const Item& it = items[id];
+        FOp op{};
+        op.kind = it.kind;
+        op.mat = it.mat;
+        if (it.kind == FOP_U1) {
+          op.t0 = reg_of[it.q0];
+          if (it.ctl >= 0) {
+            int8_t loc; int32_t pos;
+            locate(it.ctl, &loc, &pos);
+            if (loc == LOC_REG) op.ctl_reg = 1u << pos;
+            else if (loc == LOC_THR) op.ctl_thr = 1u << pos;
+            else op.ctl_out = 1ull << pos;
+          }
+        } else if (it.kind == FOP_U2) {
+          op.t0 = reg_of[it.q0];
+          op.t1 = reg_of[it.q1];
+        } else if (it.kind == FOP_D1) {
+          locate(it.q0, &op.dloc0, &op.dpos0);
+        } else {
+          locate(it.q0, &op.dloc0, &op.dpos0);
+          locate(it.q1, &op.dloc1, &op.dpos1);
+        }
+        plan->ops.push_back(op);
+      }
+      ph.op_end = (int)plan->ops.size();
+      plan->phases.push_back(ph);
+      prem = pdef;
+    }
+    st.nphase = (int)plan->phases.size() - st.phase0;
+    plan->stages.push_back(st);
+    remaining = deferred;
+  }
+  return RQ_OK;
+}
+
+int run_fused(State& s, const rq_gate* gates, size_t ngates, const double* draws, size_t ndraws, size_t* cursor,
+              rq_measurement* meas, size_t cap, size_t* produced, FusionStats* st) {
+  const double pass_bytes = 2.0 * (double)s.n * (double)s.amp_bytes();
+  size_t k = 0;
+  while (k < ngates) {
+    size_t e = k;
+    while (e < ngates && gates[e].kind != RQ_GATE_MEASURE && gates[e].kind != RQ_GATE_RESET) ++e;
+    if (e > k) {
+      if (s.nq < (uint32_t)FUSE_R_BITS) {
+        // Tiny states: the exact per-gate kernels.
+        for (size_t j = k; j < e; ++j) {
+          int has = 0;
+          rq_measurement mm;
+          int rc = apply_gate_impl(s, gates[j], 0.0, &mm, &has);
+          if (rc) return rc;
+          if (!is_non_unitary(gates[j].kind)) {
+            ++st->unitary;
+            ++st->passes;
+            st->hbm_bytes += gate_bytes(s, gates[j].kind);
+          }
+        }
+      } else {
+        FusedPlan plan;
+        int rc = plan_fused(s.nq, gates + k, e - k, fusion_config(), &plan);
+        if (rc) return rc;
+        rc = launch_fused_plan(s, plan);
+        if (rc) return rc;
+        st->unitary += plan.unitary;
+        st->passes += plan.stages.size();
+        st->hbm_bytes += pass_bytes * (double)plan.stages.size();
+        s.gate_count += plan.unitary;
+      }
+    }
+    if (e < ngates) {
+      const rq_gate& g = gates[e];
+      if (*cursor >= ndraws) return fail(RQ_ERR_CIRCUIT, "measurement needs a uniform draw but the draw list is exhausted");
+      double draw = draws[(*cursor)++];
+      if (g.q0 >= s.nq)
+        return fail(RQ_ERR_INVALID_QUBIT, "invalid qubit index " + std::to_string(g.q0) + " for " +
+                                              std::to_string(s.nq) + "-qubit system");
+      if (g.kind == RQ_GATE_MEASURE) {
+        rq_measurement mm;
+        int rc = measure_state(s, g.q0, draw, &mm);
+        if (rc) return rc;
+        if (meas && *produced < cap) meas[*produced] = mm;
+        ++*produced;
+      } else {
+        int rc = reset_state(s, g.q0, draw);
+        if (rc) return rc;
+      }
+    }
+    k = e + 1;
+  }
+  st->fused = st->unitary > st->passes ? st->unitary - st->passes : 0;
+  return RQ_OK;
+}
+
+}  // namespace rq

@@ -1,0 +1,97 @@
+// Fused execution of a circuit: gates are folded into "stages".  A stage
+// covers a set S of m tile qubits (always containing the lowest `c` qubits so
+// that every tile is made of contiguous runs in HBM).  One workgroup loads one
+// tile (the 2^m amplitudes that share the values of the other qubits) into
+// registers, applies every gate of the stage, and writes the tile back: one
+// HBM read + write per stage instead of one per gate.
+//
+// Inside a tile each thread holds R = 16 amplitudes whose index bits are the
+// 4 "register qubits" of the current phase; gates act on register qubits in
+// registers, and a phase change re-maps the tile through LDS.  Diagonal gates
+// (Rz, Phase, S, T, Z, CZ, Rzz) and the controls of CNOT never need to be in
+// S: their factor only reads index bits, which are known for every amplitude
+// (register bit, thread bit, or a per-tile constant).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "internal.h"
+
+namespace rq {
+
+constexpr int FUSE_R_BITS = 4;         // register qubits per phase
+constexpr int FUSE_R = 1 << FUSE_R_BITS;
+constexpr int FUSE_MAX_M = 12;         // largest tile (qubits): 256 threads x 16 amplitudes
+constexpr int FUSE_MAX_OUTER = 48;
+
+enum FOpKind : int32_t { FOP_U1 = 0, FOP_U2 = 1, FOP_D1 = 2, FOP_D2 = 3 };
+
+// Where an index bit of an amplitude comes from inside a phase.
+enum BitLoc : int8_t { LOC_REG = 0, LOC_THR = 1, LOC_OUT = 2 };
+
+struct FOp {
+  int32_t kind;
+  int32_t t0, t1;      // register bits of the targets (U1: t0; U2: t0 = q1, t1 = q2)
+  int32_t mat;         // offset of the matrix / diagonal in the pool (complex entries)
+  uint32_t ctl_reg;    // controls that are register bits (mask over k)
+  uint32_t ctl_thr;    // controls that are thread bits (mask over thread id)
+  uint64_t ctl_out;    // controls that are outer qubits (mask over the global index)
+  int8_t dloc0, dloc1; // diagonal bits: location kind
+  int8_t pad0, pad1;
+  int32_t dpos0, dpos1;  // bit index within that location
+};
+
+struct FPhase {
+  int32_t op_begin, op_end;
+  int32_t direct;                // 1: lanes sit on tile bits 0..5, access HBM directly
+  int32_t regt[FUSE_R_BITS];     // tile bits holding register bits 0..3
+  int32_t thrt[FUSE_MAX_M];      // tile bits holding thread-id bits 0..m-5
+};
+
+struct FStage {
+  int32_t m;            // tile qubits
+  int32_t nphase;
+  int32_t phase0;       // first phase in the phase array
+  int32_t nouter;
+  int32_t S[FUSE_MAX_M];          // global qubit of each tile bit
+  int32_t outer[FUSE_MAX_OUTER];  // global qubit of each outer bit (tile id bits)
+};
+
+struct FusionStats {
+  uint64_t unitary = 0;  // unitary input gates
+  uint64_t fused = 0;    // input gates folded (unitary - passes when positive)
+  uint64_t passes = 0;   // stage launches (full-state HBM passes)
+  double hbm_bytes = 0;  // algorithmic bytes: 2*N*s per stage
+};
+
+// Tunables of the fused schedule.
+struct FusionConfig {
+  int m = 12;      // tile qubits
+  int c = 5;       // lowest qubits always in the tile
+};
+
+// Executes gates[0..ngates) with the fused schedule; Measure/Reset gates run
+// exactly (via measure/reset on the state) between fused segments.
+int run_fused(State& s, const rq_gate* gates, size_t ngates, const double* draws, size_t ndraws, size_t* cursor,
+              rq_measurement* meas, size_t cap, size_t* produced, FusionStats* st);
+
+// Host-side plan of one fused segment (exposed for tests and the autotuner).
+struct FusedPlan {
+  std::vector<FStage> stages;
+  std::vector<FPhase> phases;
+  std::vector<FOp> ops;
+  std::vector<rq_c64> pool;
+  uint64_t unitary = 0;
+};
+int plan_fused(uint32_t nq, const rq_gate* gates, size_t ngates, const FusionConfig& cfg, FusedPlan* plan);
+
+// Device launch of a planned segment (kernels_fused.hip).
+int launch_fused_plan(State& s, const FusedPlan& plan);
+
+// Measurement helpers shared with the fused runner (state.cpp).
+int measure_state(State& s, uint32_t q, double draw, rq_measurement* out);
+int reset_state(State& s, uint32_t q, double draw);
+FusionConfig& fusion_config();
+
+}  // namespace rq

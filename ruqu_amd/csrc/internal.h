@@ -1,0 +1,81 @@
+// Internal declarations shared by the host runtime and the HIP kernel files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/ruqu_gpu.h"
+
+namespace rq {
+
+// Thread-local last error (rq_last_error) and status helpers.
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define RQ_HIP(call)                                   \
+  do {                                                 \
+    hipError_t _e = (call);                            \
+    if (_e != hipSuccess) return ::rq::hip_fail(_e, #call); \
+  } while (0)
+
+// Device-resident state vector.
+struct State {
+  uint32_t nq = 0;
+  int prec = RQ_C64;
+  size_t n = 0;              // 2^nq amplitudes
+  void* amps = nullptr;      // interleaved (re, im), 16 B or 8 B per amplitude
+  hipStream_t stream = nullptr;
+  double* red = nullptr;     // reduction partials (device)
+  size_t red_cap = 0;        // doubles
+  double* host_red = nullptr;  // pinned host mirror for small results
+  std::vector<rq_measurement> record;
+  uint64_t gate_count = 0;   // for precision_error_bound
+  void* prog = nullptr;      // device copy of the fused program
+  size_t prog_cap = 0;
+  size_t amp_bytes() const { return prec == RQ_C64 ? 16 : 8; }
+  size_t bytes() const { return n * amp_bytes(); }
+};
+
+// One gate with the exact kernels (state.rs:158-204 semantics).
+int apply_gate_impl(State& s, const rq_gate& g, double draw, rq_measurement* out, int* has_out);
+// Algorithmic bytes of the exact kernel for a gate kind.
+double gate_bytes(const State& s, int kind);
+
+// Ensure at least `doubles` of reduction scratch on the state.
+int ensure_red(State& s, size_t doubles);
+
+// ---- kernel launchers (kernels_gates.hip / kernels_reduce.hip) ----------
+// m: row-major 2x2 / 4x4 complex-f64 matrices; converted to the state's
+// precision on the host exactly as mixed_precision.rs:556-591 does.
+void launch_init_zero(State& s);
+void launch_apply_1q(State& s, uint32_t q, const rq_c64* m);
+void launch_apply_2q(State& s, uint32_t q1, uint32_t q2, const rq_c64* m);
+void launch_cnot(State& s, uint32_t c, uint32_t t);
+void launch_cz(State& s, uint32_t q1, uint32_t q2);
+void launch_swap(State& s, uint32_t q1, uint32_t q2);
+void launch_diag_2q(State& s, uint32_t q1, uint32_t q2, const rq_c64* d4);
+void launch_collapse(State& s, uint32_t q, int result, double nf64);
+void launch_scale(State& s, double f);
+void launch_copy(State& dst, const State& src);
+void launch_plain_copy(void* dst, const void* src, size_t bytes, hipStream_t st);
+
+// Reductions: results land in s.red (device); the caller reads them back.
+// Sum over amplitudes with (i & mask) == want of |a_i|^2 (FP64 accumulate).
+int reduce_masked_normsq(State& s, uint64_t mask, uint64_t want, double* out);
+// Per-term <P_k> for canonicalised Pauli strings (x, zy parity mask, nY mod 4).
+struct PauliCanon {
+  uint64_t x;
+  uint64_t zy;
+  uint32_t ny;
+};
+int reduce_pauli(State& s, const PauliCanon* terms, size_t nterms, double* out);
+int reduce_z_all(State& s, double* out);
+int reduce_inner(const State& a, const State& b, double* re, double* im);
+int launch_probabilities(State& s, double* d_out);
+int sample_shots(State& s, const double* draws, size_t shots, uint8_t* bits, uint64_t* near_ties);
+
+}  // namespace rq

@@ -1,0 +1,400 @@
+// Exact gate kernels: one HBM pass per gate, the reference's arithmetic.
+//
+// Every kernel here evaluates the same IEEE expression as ruqu-core's scalar
+// loops (state.rs:214-276, mixed_precision.rs:396-458): complex products
+// m.re*a.re - m.im*a.im / m.re*a.im + m.im*a.re and left-to-right row sums,
+// with FP contraction disabled, so f64 and f32 results equal the reference's
+// bit for bit (up to the sign of exact zeros, which the reference's dense
+// 4x4 sums can flip).  Specialised 2q kernels (CNOT, CZ, SWAP, diagonal) skip
+// only the products by exact zeros and ones.
+//
+// Layout: interleaved (re, im); f32 kernels move 16 B per load (two
+// amplitudes, a float4) wherever both amplitudes share the pair structure.
+#include "internal.h"
+
+#pragma clang fp contract(off)
+
+namespace rq {
+namespace {
+
+constexpr int BLOCK = 256;
+
+template <class T>
+struct Cx {
+  T re, im;
+};
+
+template <class T>
+__device__ __forceinline__ Cx<T> cmul(Cx<T> m, Cx<T> a) {
+  return Cx<T>{m.re * a.re - m.im * a.im, m.re * a.im + m.im * a.re};
+}
+template <class T>
+__device__ __forceinline__ Cx<T> cadd(Cx<T> a, Cx<T> b) {
+  return Cx<T>{a.re + b.re, a.im + b.im};
+}
+
+template <class T>
+struct M2 {
+  Cx<T> m00, m01, m10, m11;
+};
+template <class T>
+struct M4 {
+  Cx<T> m[16];
+};
+
+// Insert a zero bit at position q of p.
+__device__ __forceinline__ uint64_t ins0(uint64_t p, uint32_t q) {
+  uint64_t lo = p & ((1ull << q) - 1);
+  return ((p >> q) << (q + 1)) | lo;
+}
+
+// ---- 1q, f32, q >= 1: each group = two adjacent pairs (float4 each side) --
+template <int U>
+__global__ __launch_bounds__(BLOCK) void k_1q_f32_v2(float4* __restrict__ a, uint32_t q, uint64_t ngroups,
+                                                     M2<float> m) {
+  const uint64_t g0 = (uint64_t)blockIdx.x * U * BLOCK + threadIdx.x;
+  float4 x[U], y[U];
+  uint64_t ii[U], jj[U];
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    uint64_t g = g0 + (uint64_t)k * BLOCK;
+    if (g < ngroups) {
+      uint64_t i = ins0(2 * g, q);
+      ii[k] = i >> 1;
+      jj[k] = (i + (1ull << q)) >> 1;
+      x[k] = a[ii[k]];
+      y[k] = a[jj[k]];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    uint64_t g = g0 + (uint64_t)k * BLOCK;
+    if (g < ngroups) {
+      Cx<float> a0{x[k].x, x[k].y}, a1{x[k].z, x[k].w};
+      Cx<float> b0{y[k].x, y[k].y}, b1{y[k].z, y[k].w};
+      Cx<float> o0 = cadd(cmul(m.m00, a0), cmul(m.m01, b0));
+      Cx<float> o1 = cadd(cmul(m.m00, a1), cmul(m.m01, b1));
+      Cx<float> p0 = cadd(cmul(m.m10, a0), cmul(m.m11, b0));
+      Cx<float> p1 = cadd(cmul(m.m10, a1), cmul(m.m11, b1));
+      a[ii[k]] = make_float4(o0.re, o0.im, o1.re, o1.im);
+      a[jj[k]] = make_float4(p0.re, p0.im, p1.re, p1.im);
+    }
+  }
+}
+
+// ---- 1q, f32, q == 0: one float4 holds the pair (a, b) --------------------
+template <int U>
+__global__ __launch_bounds__(BLOCK) void k_1q_f32_q0(float4* __restrict__ a, uint64_t npairs, M2<float> m) {
+  const uint64_t g0 = (uint64_t)blockIdx.x * U * BLOCK + threadIdx.x;
+  float4 x[U];
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    uint64_t g = g0 + (uint64_t)k * BLOCK;
+    if (g < npairs) x[k] = a[g];
+  }
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    uint64_t g = g0 + (uint64_t)k * BLOCK;
+    if (g < npairs) {
+      Cx<float> av{x[k].x, x[k].y}, bv{x[k].z, x[k].w};
+      Cx<float> o = cadd(cmul(m.m00, av), cmul(m.m01, bv));
+      Cx<float> p = cadd(cmul(m.m10, av), cmul(m.m11, bv));
+      a[g] = make_float4(o.re, o.im, p.re, p.im);
+    }
+  }
+}
+
+// ---- 1q, f64: one double2 per amplitude -----------------------------------
+template <int U>
+__global__ __launch_bounds__(BLOCK) void k_1q_f64(double2* __restrict__ a, uint32_t q, uint64_t npairs,
+                                                  M2<double> m) {
+  const uint64_t g0 = (uint64_t)blockIdx.x * U * BLOCK + threadIdx.x;
+  double2 x[U], y[U];
+  uint64_t ii[U];
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    uint64_t g = g0 + (uint64_t)k * BLOCK;
+    if (g < npairs) {
+      ii[k] = ins0(g, q);
+      x[k] = a[ii[k]];
+      y[k] = a[ii[k] + (1ull << q)];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    uint64_t g = g0 + (uint64_t)k * BLOCK;
+    if (g < npairs) {
+      Cx<double> av{x[k].x, x[k].y}, bv{y[k].x, y[k].y};
+      Cx<double> o = cadd(cmul(m.m00, av), cmul(m.m01, bv));
+      Cx<double> p = cadd(cmul(m.m10, av), cmul(m.m11, bv));
+      a[ii[k]] = make_double2(o.re, o.im);
+      a[ii[k] + (1ull << q)] = make_double2(p.re, p.im);
+    }
+  }
+}
+
+// ---- dense 2q (state.rs:238-276): one quad per thread ---------------------
+template <class T, class V>
+__global__ __launch_bounds__(BLOCK) void k_2q_dense(V* __restrict__ a, uint32_t q1, uint32_t q2, uint64_t nquads,
+                                                    M4<T> m) {
+  uint64_t g = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (g >= nquads) return;
+  uint32_t lo = q1 < q2 ? q1 : q2, hi = q1 < q2 ? q2 : q1;
+  uint64_t base = ins0(ins0(g, lo), hi);
+  uint64_t b1 = 1ull << q1, b2 = 1ull << q2;
+  uint64_t idx[4] = {base, base | b2, base | b1, base | b1 | b2};
+  Cx<T> v[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    V t = a[idx[r]];
+    v[r] = Cx<T>{t.x, t.y};
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    Cx<T> s = cadd(cadd(cadd(cmul(m.m[r * 4 + 0], v[0]), cmul(m.m[r * 4 + 1], v[1])), cmul(m.m[r * 4 + 2], v[2])),
+                   cmul(m.m[r * 4 + 3], v[3]));
+    V o;
+    o.x = s.re;
+    o.y = s.im;
+    a[idx[r]] = o;
+  }
+}
+
+// ---- CNOT(c, t): swap the t=0 / t=1 amplitudes where c=1 ------------------
+template <class V>
+__global__ __launch_bounds__(BLOCK) void k_cnot(V* __restrict__ a, uint32_t c, uint32_t t, uint64_t nquads) {
+  uint64_t g = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (g >= nquads) return;
+  uint32_t lo = c < t ? c : t, hi = c < t ? t : c;
+  uint64_t base = ins0(ins0(g, lo), hi) | (1ull << c);
+  V x = a[base], y = a[base | (1ull << t)];
+  a[base] = y;
+  a[base | (1ull << t)] = x;
+}
+
+// ---- SWAP(q1, q2): exchange |01> and |10> ---------------------------------
+template <class V>
+__global__ __launch_bounds__(BLOCK) void k_swap(V* __restrict__ a, uint32_t q1, uint32_t q2, uint64_t nquads) {
+  uint64_t g = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (g >= nquads) return;
+  uint32_t lo = q1 < q2 ? q1 : q2, hi = q1 < q2 ? q2 : q1;
+  uint64_t base = ins0(ins0(g, lo), hi);
+  uint64_t i = base | (1ull << q2), j = base | (1ull << q1);
+  V x = a[i], y = a[j];
+  a[i] = y;
+  a[j] = x;
+}
+
+// ---- CZ: (-1 + 0i) * a on |11> --------------------------------------------
+template <class T, class V>
+__global__ __launch_bounds__(BLOCK) void k_cz(V* __restrict__ a, uint32_t q1, uint32_t q2, uint64_t nquads) {
+  uint64_t g = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (g >= nquads) return;
+  uint32_t lo = q1 < q2 ? q1 : q2, hi = q1 < q2 ? q2 : q1;
+  uint64_t i = ins0(ins0(g, lo), hi) | (1ull << q1) | (1ull << q2);
+  V x = a[i];
+  Cx<T> r = cmul(Cx<T>{T(-1), T(0)}, Cx<T>{x.x, x.y});
+  x.x = r.re;
+  x.y = r.im;
+  a[i] = x;
+}
+
+// ---- diagonal 2q (Rzz): a_i *= d[b1*2 + b2], f32 float4 / f64 double2 ----
+__global__ __launch_bounds__(BLOCK) void k_diag2_f32(float4* __restrict__ a, uint32_t q1, uint32_t q2,
+                                                     uint64_t nvec, M4<float> d) {
+  uint64_t g = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (g >= nvec) return;
+  float4 x = a[g];
+  uint64_t i0 = 2 * g, i1 = 2 * g + 1;
+  int k0 = (int)(((i0 >> q1) & 1) * 2 + ((i0 >> q2) & 1));
+  int k1 = (int)(((i1 >> q1) & 1) * 2 + ((i1 >> q2) & 1));
+  Cx<float> r0 = cmul(d.m[k0 * 5], Cx<float>{x.x, x.y});
+  Cx<float> r1 = cmul(d.m[k1 * 5], Cx<float>{x.z, x.w});
+  a[g] = make_float4(r0.re, r0.im, r1.re, r1.im);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_diag2_f64(double2* __restrict__ a, uint32_t q1, uint32_t q2, uint64_t n,
+                                                     M4<double> d) {
+  uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  double2 x = a[i];
+  int k = (int)(((i >> q1) & 1) * 2 + ((i >> q2) & 1));
+  Cx<double> r = cmul(d.m[k * 5], Cx<double>{x.x, x.y});
+  a[i] = make_double2(r.re, r.im);
+}
+
+// ---- |0..0> -----------------------------------------------------------------
+template <class V>
+__global__ __launch_bounds__(BLOCK) void k_init_zero(V* __restrict__ a, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  V z{};
+  if (i == 0) z.x = 1;
+  a[i] = z;
+}
+
+// ---- measurement collapse (state.rs:310-317, mixed_precision.rs:499-506) ---
+template <class T, class V>
+__global__ __launch_bounds__(BLOCK) void k_collapse(V* __restrict__ a, uint32_t q, int result, T nf, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  V x = a[i];
+  int one = (int)((i >> q) & 1);
+  if (one == result) {
+    x.x = x.x * nf;
+    x.y = x.y * nf;
+  } else {
+    x.x = 0;
+    x.y = 0;
+  }
+  a[i] = x;
+}
+
+// ---- scale every amplitude (normalize, state.rs:419-421) -------------------
+template <class T, class V>
+__global__ __launch_bounds__(BLOCK) void k_scale(V* __restrict__ a, T f, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  V x = a[i];
+  x.x = x.x * f;
+  x.y = x.y * f;
+  a[i] = x;
+}
+
+// ---- plain 16 B copy (bandwidth reference and state copies) ----------------
+template <int U>
+__global__ __launch_bounds__(BLOCK) void k_copy16(float4* __restrict__ dst, const float4* __restrict__ src,
+                                                  uint64_t nvec) {
+  const uint64_t g0 = (uint64_t)blockIdx.x * U * BLOCK + threadIdx.x;
+  float4 x[U];
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    uint64_t g = g0 + (uint64_t)k * BLOCK;
+    if (g < nvec) x[k] = src[g];
+  }
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    uint64_t g = g0 + (uint64_t)k * BLOCK;
+    if (g < nvec) dst[g] = x[k];
+  }
+}
+
+inline dim3 grid_for(uint64_t items, uint64_t per_block) { return dim3((unsigned)((items + per_block - 1) / per_block)); }
+
+template <class T>
+M2<T> to_m2(const rq_c64* m) {
+  M2<T> r;
+  r.m00 = Cx<T>{(T)m[0].re, (T)m[0].im};
+  r.m01 = Cx<T>{(T)m[1].re, (T)m[1].im};
+  r.m10 = Cx<T>{(T)m[2].re, (T)m[2].im};
+  r.m11 = Cx<T>{(T)m[3].re, (T)m[3].im};
+  return r;
+}
+template <class T>
+M4<T> to_m4(const rq_c64* m) {
+  M4<T> r;
+  for (int k = 0; k < 16; ++k) r.m[k] = Cx<T>{(T)m[k].re, (T)m[k].im};
+  return r;
+}
+
+constexpr int U1 = 4;  // pairs (or pair groups) per thread in the 1q kernels
+
+}  // namespace
+
+void launch_init_zero(State& s) {
+  if (s.prec == RQ_C64)
+    hipLaunchKernelGGL(k_init_zero<double2>, grid_for(s.n, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, s.n);
+  else
+    hipLaunchKernelGGL(k_init_zero<float2>, grid_for(s.n, BLOCK), dim3(BLOCK), 0, s.stream, (float2*)s.amps, s.n);
+}
+
+void launch_apply_1q(State& s, uint32_t q, const rq_c64* m) {
+  if (s.prec == RQ_C64) {
+    uint64_t np = s.n / 2;
+    hipLaunchKernelGGL(k_1q_f64<U1>, grid_for(np, (uint64_t)BLOCK * U1), dim3(BLOCK), 0, s.stream,
+                       (double2*)s.amps, q, np, to_m2<double>(m));
+  } else if (q == 0) {
+    uint64_t np = s.n / 2;
+    hipLaunchKernelGGL(k_1q_f32_q0<U1>, grid_for(np, (uint64_t)BLOCK * U1), dim3(BLOCK), 0, s.stream,
+                       (float4*)s.amps, np, to_m2<float>(m));
+  } else {
+    uint64_t ng = s.n / 4;
+    hipLaunchKernelGGL(k_1q_f32_v2<U1>, grid_for(ng, (uint64_t)BLOCK * U1), dim3(BLOCK), 0, s.stream,
+                       (float4*)s.amps, q, ng, to_m2<float>(m));
+  }
+}
+
+void launch_apply_2q(State& s, uint32_t q1, uint32_t q2, const rq_c64* m) {
+  uint64_t nq4 = s.n / 4;
+  if (s.prec == RQ_C64)
+    hipLaunchKernelGGL((k_2q_dense<double, double2>), grid_for(nq4, BLOCK), dim3(BLOCK), 0, s.stream,
+                       (double2*)s.amps, q1, q2, nq4, to_m4<double>(m));
+  else
+    hipLaunchKernelGGL((k_2q_dense<float, float2>), grid_for(nq4, BLOCK), dim3(BLOCK), 0, s.stream,
+                       (float2*)s.amps, q1, q2, nq4, to_m4<float>(m));
+}
+
+void launch_cnot(State& s, uint32_t c, uint32_t t) {
+  uint64_t nq4 = s.n / 4;
+  if (s.prec == RQ_C64)
+    hipLaunchKernelGGL(k_cnot<double2>, grid_for(nq4, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, c, t, nq4);
+  else
+    hipLaunchKernelGGL(k_cnot<float2>, grid_for(nq4, BLOCK), dim3(BLOCK), 0, s.stream, (float2*)s.amps, c, t, nq4);
+}
+
+void launch_swap(State& s, uint32_t q1, uint32_t q2) {
+  uint64_t nq4 = s.n / 4;
+  if (s.prec == RQ_C64)
+    hipLaunchKernelGGL(k_swap<double2>, grid_for(nq4, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, q1, q2, nq4);
+  else
+    hipLaunchKernelGGL(k_swap<float2>, grid_for(nq4, BLOCK), dim3(BLOCK), 0, s.stream, (float2*)s.amps, q1, q2, nq4);
+}
+
+void launch_cz(State& s, uint32_t q1, uint32_t q2) {
+  uint64_t nq4 = s.n / 4;
+  if (s.prec == RQ_C64)
+    hipLaunchKernelGGL((k_cz<double, double2>), grid_for(nq4, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, q1,
+                       q2, nq4);
+  else
+    hipLaunchKernelGGL((k_cz<float, float2>), grid_for(nq4, BLOCK), dim3(BLOCK), 0, s.stream, (float2*)s.amps, q1, q2,
+                       nq4);
+}
+
+void launch_diag_2q(State& s, uint32_t q1, uint32_t q2, const rq_c64* d4) {
+  if (s.prec == RQ_C64) {
+    hipLaunchKernelGGL(k_diag2_f64, grid_for(s.n, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, q1, q2, s.n,
+                       to_m4<double>(d4));
+  } else {
+    uint64_t nv = s.n / 2;
+    hipLaunchKernelGGL(k_diag2_f32, grid_for(nv, BLOCK), dim3(BLOCK), 0, s.stream, (float4*)s.amps, q1, q2, nv,
+                       to_m4<float>(d4));
+  }
+}
+
+void launch_collapse(State& s, uint32_t q, int result, double nf64) {
+  if (s.prec == RQ_C64)
+    hipLaunchKernelGGL((k_collapse<double, double2>), grid_for(s.n, BLOCK), dim3(BLOCK), 0, s.stream,
+                       (double2*)s.amps, q, result, nf64, s.n);
+  else
+    hipLaunchKernelGGL((k_collapse<float, float2>), grid_for(s.n, BLOCK), dim3(BLOCK), 0, s.stream, (float2*)s.amps,
+                       q, result, (float)nf64, s.n);
+}
+
+void launch_scale(State& s, double f) {
+  if (s.prec == RQ_C64)
+    hipLaunchKernelGGL((k_scale<double, double2>), grid_for(s.n, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps,
+                       f, s.n);
+  else
+    hipLaunchKernelGGL((k_scale<float, float2>), grid_for(s.n, BLOCK), dim3(BLOCK), 0, s.stream, (float2*)s.amps,
+                       (float)f, s.n);
+}
+
+void launch_plain_copy(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  uint64_t nv = bytes / 16;
+  hipLaunchKernelGGL(k_copy16<4>, grid_for(nv, (uint64_t)BLOCK * 4), dim3(BLOCK), 0, st, (float4*)dst,
+                     (const float4*)src, nv);
+}
+
+void launch_copy(State& dst, const State& src) { launch_plain_copy(dst.amps, src.amps, src.bytes(), dst.stream); }
+
+}  // namespace rq

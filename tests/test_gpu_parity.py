@@ -1,0 +1,265 @@
+"""Device parity: libruqu_gpu.so against the oracle on the same seeded inputs.
+
+Bars: the exact per-gate kernels are bit-identical to the oracle (f64 and
+f32, compared with ==, which ignores the sign of zero); the fused schedule
+and every reduction (FP64 accumulation in a different order) are within
+max-abs 1e-12 in f64 mode and 1e-5 in f32 mode against the FP64 oracle;
+measured bits are identical given the same draws except where a draw lies
+within 1e-12 of its threshold (counted and reported as near ties).
+"""
+import numpy as np
+import pytest
+
+from ruqu_amd import (CircuitError, Gate, Hamiltonian, InvalidQubitIndex, InvalidStateVector, PauliString,
+                      QuantumCircuit, QuantumState, QubitLimitExceeded, SimConfig, Simulator)
+from ruqu_amd import workloads
+from tests.helpers import ALL_1Q, ALL_2Q, max_abs, mixed_circuit, orc, random_state, random_unitary
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"c64": 1e-12, "c32": 1e-5}
+NP = {"c64": np.complex128, "c32": np.complex64}
+
+
+def flush(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("prec", ["c64", "c32"])
+def test_every_gate_bit_exact(prec):
+    n = 10
+    flush("every gate %s n=%d" % (prec, n))
+    psi = random_state(n, 1, NP[prec])
+    st = QuantumState.from_amplitudes(psi, n, prec)
+    for kind in ALL_1Q:
+        for q in (0, 1, 3, 5, 9):
+            g = Gate(kind, q, 0, 0.731)
+            st.set_amplitudes(psi)
+            st.apply_gate(g)
+            want = psi.copy()
+            orc.apply_gate(want, n, QuantumCircuit(n).add_gate(g).to_array()[0])
+            assert np.array_equal(st.state_vector(), want), (kind, q)
+    for kind in ALL_2Q:
+        for q1, q2 in [(0, 1), (1, 0), (0, 9), (9, 0), (3, 7), (7, 3), (4, 5)]:
+            g = Gate(kind, q1, q2, 1.37)
+            st.set_amplitudes(psi)
+            st.apply_gate(g)
+            want = psi.copy()
+            orc.apply_gate(want, n, QuantumCircuit(n).add_gate(g).to_array()[0])
+            assert np.array_equal(st.state_vector(), want), (kind, q1, q2)
+    u = random_unitary(2, 4)
+    st.set_amplitudes(psi)
+    st.apply_gate(Gate.Unitary1Q(6, u))
+    want = psi.copy()
+    orc.apply_gate(want, n, QuantumCircuit(n).add_gate(Gate.Unitary1Q(6, u)).to_array()[0])
+    assert np.array_equal(st.state_vector(), want)
+
+
+@pytest.mark.parametrize("prec", ["c64", "c32"])
+def test_dense_matrices_bit_exact(prec):
+    n = 9
+    flush("dense 1q/2q %s" % prec)
+    psi = random_state(n, 2, NP[prec])
+    st = QuantumState.from_amplitudes(psi, n, prec)
+    for q in range(n):
+        m = random_unitary(2, q)
+        st.set_amplitudes(psi)
+        st.apply_single_qubit_gate(q, m)
+        want = psi.copy()
+        orc.apply_1q(want, n, q, m)
+        assert np.array_equal(st.state_vector(), want)
+    for q1, q2 in [(0, 1), (1, 0), (2, 8), (8, 2), (5, 6)]:
+        m = random_unitary(4, q1 * 9 + q2)
+        st.set_amplitudes(psi)
+        st.apply_two_qubit_gate(q1, q2, m)
+        want = psi.copy()
+        orc.apply_2q(want, n, q1, q2, m)
+        assert np.array_equal(st.state_vector(), want)
+
+
+@pytest.mark.parametrize("prec", ["c64", "c32"])
+@pytest.mark.parametrize("n", [2, 5, 12])
+def test_circuit_exact_path_bit_exact(prec, n):
+    flush("exact circuit %s n=%d" % (prec, n))
+    c = mixed_circuit(n, 6, 100 + n)
+    res = Simulator.run_with_config(c, SimConfig(precision=prec))
+    want, _, cnt = orc.run_circuit(n, c.to_array(), f32=(prec == "c32"))
+    assert np.array_equal(res.state.state_vector(), want)
+    assert res.metrics.gate_count == cnt
+
+
+@pytest.mark.parametrize("prec", ["c64", "c32"])
+@pytest.mark.parametrize("n", [4, 5, 7, 9, 11, 12, 13, 16])
+def test_circuit_fused_within_tolerance(prec, n):
+    flush("fused circuit %s n=%d" % (prec, n))
+    c = mixed_circuit(n, 5, 200 + n)
+    res = Simulator.run_with_config(c, SimConfig(precision=prec, fusion=True))
+    want, _, cnt = orc.run_circuit(n, c.to_array())
+    err = max_abs(res.state.state_vector(), want)
+    flush("  max-abs vs f64 oracle %.3e, passes %d for %d gates" % (err, res.metrics.passes, cnt))
+    assert err < TOL[prec]
+    assert res.metrics.gate_count == cnt
+
+
+@pytest.mark.parametrize("prec", ["c64", "c32"])
+@pytest.mark.parametrize("n", [14, 18])
+def test_random_circuit_workload_fused(prec, n):
+    flush("random_circuit fused %s n=%d" % (prec, n))
+    c = workloads.random_circuit(n, 8, 7)
+    res = Simulator.run_with_config(c, SimConfig(precision=prec, fusion=True))
+    want, _, _ = orc.run_circuit(n, c.to_array())
+    err = max_abs(res.state.state_vector(), want)
+    flush("  max-abs %.3e passes %d" % (err, res.metrics.passes))
+    assert err < TOL[prec]
+
+
+def test_config1_20q_c64():
+    """20-qubit random circuit, depth 20, complex-f64: all amplitudes, <Z_i>
+    for every qubit and 10^4 measured shots."""
+    n, depth = 20, 20
+    flush("config-1 20q depth 20")
+    c = workloads.random_circuit(n, depth, 1)
+    want, _, _ = orc.run_circuit(n, c.to_array())
+    exact = Simulator.run_with_config(c, SimConfig(precision="c64"))
+    got = exact.state.state_vector()
+    assert np.array_equal(got, want)
+    fused = Simulator.run_with_config(c, SimConfig(precision="c64", fusion=True))
+    ferr = max_abs(fused.state.state_vector(), want)
+    flush("  fused max-abs %.3e (%d passes for %d gates)" % (ferr, fused.metrics.passes, fused.metrics.gate_count))
+    assert ferr < 1e-12
+    z = exact.state.expectation_z_all()
+    zw = np.array([orc.expectation(want, n, [(q, 3)]) for q in range(n)])
+    assert np.max(np.abs(z - zw)) < 1e-12
+    shots = 10_000
+    draws = workloads.uniforms(workloads.shot_seed(1), shots * n).reshape(shots, n)
+    bits, ties = exact.state.sample_shots(draws)
+    assert bits.shape == (shots, n)
+    flush("  near ties %d" % ties)
+    checked = 64
+    for s in range(checked):
+        a = want.copy()
+        res, _ = orc.measure_all(a, n, draws[s])
+        if ties == 0:
+            assert list(bits[s]) == list(res), s
+
+
+@pytest.mark.parametrize("prec", ["c64", "c32"])
+def test_sample_shots_match_measure_all(prec):
+    n = 11
+    flush("shots %s" % prec)
+    c = mixed_circuit(n, 4, 3)
+    res = Simulator.run_with_config(c, SimConfig(precision=prec))
+    amps = res.state.state_vector()
+    shots = 2000
+    draws = workloads.uniforms(77, shots * n).reshape(shots, n)
+    bits, ties = res.state.sample_shots(draws)
+    mismatch = 0
+    for s in range(shots):
+        a = amps.copy()
+        r, _ = orc.measure_all(a, n, draws[s])
+        mismatch += int(list(bits[s]) != list(r))
+    flush("  mismatches %d near ties %d" % (mismatch, ties))
+    if prec == "c64":
+        assert mismatch <= ties
+    else:
+        # f32: the reference accumulates p0 in f32 sequentially; ours is FP64.
+        assert mismatch <= max(ties, 5)
+
+
+def test_run_shots_histogram_semantics():
+    n = 3
+    c = QuantumCircuit(n).h(0).cnot(0, 1)
+    r = Simulator.run_shots(c, 500, seed=5)
+    assert sum(r.counts.values()) == 500
+    for key in r.counts:
+        assert key[0] == key[1] and key[2] is False
+    c2 = QuantumCircuit(n).h(0).cnot(0, 1).measure(0)
+    r2 = Simulator.run_shots(c2, 50, seed=5)
+    for key in r2.counts:
+        assert key[1] is False  # only qubit 0 recorded (simulator.rs:139-145)
+
+
+@pytest.mark.parametrize("prec", ["c64", "c32"])
+def test_measure_and_reset(prec):
+    n = 8
+    flush("measure/reset %s" % prec)
+    psi = random_state(n, 5, NP[prec])
+    for q in range(n):
+        for draw in (0.05, 0.5, 0.95):
+            st = QuantumState.from_amplitudes(psi, n, prec)
+            out = st.measure(q, draw)
+            a = psi.copy()
+            r, p = orc.measure(a, n, q, draw)
+            assert out.result == r
+            assert abs(out.probability - p) < TOL[prec]
+            assert max_abs(st.state_vector(), a) < TOL[prec]
+            st2 = QuantumState.from_amplitudes(psi, n, prec)
+            st2.reset_qubit(q, draw)
+            assert st2.probability_of_qubit(q) < TOL[prec]
+    st = QuantumState.from_amplitudes(psi, n, prec)
+    st.measure(0, 0.3)
+    st.measure(1, 0.7)
+    rec = st.measurement_record()
+    assert [m.qubit for m in rec] == [0, 1]
+
+
+@pytest.mark.parametrize("prec", ["c64", "c32"])
+def test_observables(prec):
+    n = 10
+    flush("observables %s" % prec)
+    psi = random_state(n, 9, NP[prec])
+    st = QuantumState.from_amplitudes(psi, n, prec)
+    p64 = psi.astype(np.complex128)
+    rng = np.random.default_rng(3)
+    for _ in range(25):
+        w = int(rng.integers(0, 6))
+        ops = [(int(rng.integers(n)), int(rng.integers(4))) for _ in range(w)]
+        got = st.expectation_value(PauliString(ops))
+        assert abs(got - orc.expectation(p64, n, ops)) < 1e-13
+    terms = workloads.random_pauli_hamiltonian(n, 40, 4, 5)
+    h = Hamiltonian([(cf, ops) for cf, ops in terms], n)
+    assert abs(st.expectation_hamiltonian(h) - orc.expectation_hamiltonian(p64, n, terms)) < 1e-12
+    z = st.expectation_z_all()
+    for q in range(n):
+        assert abs(z[q] - orc.expectation(p64, n, [(q, 3)])) < 1e-13
+    pr = st.probabilities()
+    assert np.array_equal(pr, orc.probabilities(psi, n))
+    for q in range(n):
+        assert abs(st.probability_of_qubit(q) - orc.probability_of_qubit(p64, n, q)) < 1e-14
+    other = random_state(n, 10, NP[prec])
+    st2 = QuantumState.from_amplitudes(other, n, prec)
+    assert abs(st.fidelity(st2) - orc.fidelity(p64, other.astype(np.complex128), n)) < 1e-13
+    scaled = (psi * 1.7).astype(NP[prec])
+    st3 = QuantumState.from_amplitudes(scaled, n, prec)
+    st3.normalize()
+    a = scaled.astype(np.complex128)
+    orc.normalize(a, n)
+    assert max_abs(st3.state_vector(), a) < TOL[prec]
+
+
+def test_errors_mirror_quantum_error():
+    with pytest.raises(CircuitError):
+        QuantumState.new(0)
+    QuantumState.new_with_seed(0, 1)  # accepted for f64 (state.rs:55-71)
+    with pytest.raises(CircuitError):
+        QuantumState.new_with_seed(0, 1, "c32")
+    with pytest.raises(QubitLimitExceeded):
+        QuantumState.new(60)
+    st = QuantumState.new(2)
+    with pytest.raises(InvalidQubitIndex):
+        st.apply_gate(Gate.H(5))
+    with pytest.raises(CircuitError):
+        st.apply_gate(Gate.CNOT(1, 1))
+    with pytest.raises(InvalidStateVector):
+        QuantumState.from_amplitudes(np.zeros(3, dtype=np.complex128), 2)
+    with pytest.raises(InvalidQubitIndex):
+        st.expectation_value(PauliString([(4, 1)]))
+
+
+def test_f32_precision_error_bound():  # mixed_precision.rs:729-740
+    st = QuantumState.new_with_seed(2, 42, "c32")
+    assert st.precision_error_bound() == 0.0
+    st.apply_gate(Gate.H(0))
+    st.apply_gate(Gate.CNOT(0, 1))
+    b = st.precision_error_bound()
+    assert 0.0 < b < 1e-5
