@@ -185,6 +185,20 @@ int rq_normalize(rq_state* s);                                  /* state.rs:415-
 int rq_fidelity(const rq_state* a, const rq_state* b, double* out); /* state.rs:426-435 */
 int rq_norm_sq(const rq_state* s, double* out);
 
+/* ---- fused schedule (no reference counterpart: the reference's optimizer
+ * only folds adjacent 1q gates, optimizer.rs:46-94, and is not called by
+ * Simulator) ------------------------------------------------------------ */
+/* Tile size m (qubits per LDS/register tile, 4..12) and count c of lowest
+ * qubits kept in every tile; applies to later rq_run_circuit(fusion=1). */
+int rq_fusion_configure(int m, int c);
+/* Host-only: schedule size of gates[0..n) (no Measure/Reset). */
+int rq_fusion_plan_info(uint32_t num_qubits, const rq_gate* gates, size_t ngates, int m, int c, uint64_t* stages,
+                        uint64_t* phases, uint64_t* ops);
+/* Host-only: executes the fused schedule on host amplitudes with the device
+ * kernel's data mapping (planner check without a GPU). */
+int rq_fusion_emulate(uint32_t num_qubits, int precision, const rq_gate* gates, size_t ngates, int m, int c,
+                      void* host_amps);
+
 /* ---- timing helpers for the measurement harness ------------------------ */
 /* Time `reps` back-to-back launches of the exact 1q kernel on `qubit` with
  * matrix m (HIP events on the state's stream); *avg_ms = mean per launch. */
