@@ -185,6 +185,45 @@ int rq_normalize(rq_state* s);                                  /* state.rs:415-
 int rq_fidelity(const rq_state* a, const rq_state* b, double* out); /* state.rs:426-435 */
 int rq_norm_sq(const rq_state* s, double* out);
 
+/* ---- QAOA MaxCut (crates/ruqu-algorithms/src/qaoa.rs) ------------------- */
+/* qaoa.rs:34-40 Graph: num_nodes qubits, edges (ei[k], ej[k]) with weight w[k]. */
+typedef struct {
+  uint32_t num_nodes;
+  uint32_t num_edges;
+  const uint32_t* ei;
+  const uint32_t* ej;
+  const double* w;
+} rq_graph;
+typedef struct rq_qaoa rq_qaoa; /* one instance: graph + its device state */
+/* qaoa.rs:96-110 QaoaResult (angles and history are separate outputs). */
+typedef struct {
+  double best_cut_value;
+  uint64_t best_index;   /* best_bitstring[v] = (best_index >> v) & 1 */
+  uint32_t iterations;   /* energy_history length */
+  int32_t converged;
+  uint64_t evaluations;  /* evaluate_qaoa_cost calls */
+  double device_ms;      /* HIP-event time of all evaluations */
+  double wall_ms;
+} rq_qaoa_result;
+/* Validates the graph like Graph::add_edge (qaoa.rs:56-60; node index out of
+ * range -> RQ_ERR_INVALID_QUBIT instead of a panic) and allocates the state. */
+int rq_qaoa_create(const rq_graph* g, int precision, rq_qaoa** out);
+void rq_qaoa_destroy(rq_qaoa* q);
+/* The state left by the last evaluation (owned by the instance). */
+rq_state* rq_qaoa_state(rq_qaoa* q);
+/* evaluate_qaoa_cost (qaoa.rs:179-201).  mode 0: the reference's arithmetic
+ * (bit-identical f64 amplitudes); mode 1: fused mixer + one-pass energy. */
+int rq_qaoa_cost(rq_qaoa* q, const double* gammas, const double* betas, uint32_t p, int mode, double* cost);
+/* run_qaoa (qaoa.rs:217-308).  init_* may be null (0.5 each, qaoa.rs:221-222);
+ * history has room for max_iter values. */
+int rq_qaoa_run(rq_qaoa* q, uint32_t p, uint32_t max_iter, double lr, int mode, const double* init_gammas,
+                const double* init_betas, double* gammas_out, double* betas_out, double* history,
+                rq_qaoa_result* res);
+/* argmax_i |a_i|^2 with ties to the larger index (qaoa.rs:249-255). */
+int rq_argmax_probability(const rq_state* s, uint64_t* index, double* prob);
+/* <prod_{q in masks[k]} Z_q> for each mask, one FP64 pass per 32 masks. */
+int rq_expectation_zmasks(const rq_state* s, const uint64_t* masks, size_t n, double* out);
+
 /* ---- fused schedule (no reference counterpart: the reference's optimizer
  * only folds adjacent 1q gates, optimizer.rs:46-94, and is not called by
  * Simulator) ------------------------------------------------------------ */
@@ -198,6 +237,13 @@ int rq_fusion_plan_info(uint32_t num_qubits, const rq_gate* gates, size_t ngates
  * kernel's data mapping (planner check without a GPU). */
 int rq_fusion_emulate(uint32_t num_qubits, int precision, const rq_gate* gates, size_t ngates, int m, int c,
                       void* host_amps);
+
+/* Process-wide tuning knobs (no reference counterpart; set by the autotuner):
+ * key 0 = streaming kernels' amplitude groups per thread (1, 2, 4, 8),
+ * key 1 = non-temporal HBM access (0/1), key 2 = fused tile qubits m,
+ * key 3 = fused low qubits c. */
+int rq_tune_set(int key, int value);
+int rq_tune_get(int key);
 
 /* ---- timing helpers for the measurement harness ------------------------ */
 /* Time `reps` back-to-back launches of the exact 1q kernel on `qubit` with

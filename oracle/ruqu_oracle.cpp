@@ -40,8 +40,16 @@ inline C64 operator*(C64 a, C64 b) {
 inline C64 scale(C64 a, double s) { return C64{a.re * s, a.im * s}; }  // types.rs:129-138
 inline C64 conj(C64 a) { return C64{a.re, -a.im}; }                     // types.rs:42-47
 inline double norm_sq(C64 a) { return a.re * a.re + a.im * a.im; }     // types.rs:32-34
+// (cos th, sin th) of one argument through one glibc sincos call.  Rust's
+// f64::cos/sin lower to LLVM intrinsics, and LLVM turns a cos/sin pair of one
+// argument into a single `sincos` libcall on GNU targets; glibc's sincos and
+// its separate sin/cos can differ in the last bit, so both the oracle and the
+// engine call sincos explicitly wherever the reference evaluates such a pair.
+inline void cos_sin(double th, double* c, double* s) { sincos(th, s, c); }
 inline C64 from_polar(double r, double th) {                            // types.rs:24-29
-  return C64{r * std::cos(th), r * std::sin(th)};
+  double c, s;
+  cos_sin(th, &c, &s);
+  return C64{r * c, r * s};
 }
 
 // mixed_precision.rs:33-162 — Complex32, same op order in f32.
@@ -118,15 +126,16 @@ bool matrix_1q(const orc_gate& g, C64 m[2][2]) {
       return true;
     }
     case G_RX: {  // :112-117
-      double half = g.theta / 2.0;
-      C64 c{std::cos(half), 0.0};
-      C64 s{0.0, -std::sin(half)};
+      double half = g.theta / 2.0, ch, sh;
+      cos_sin(half, &ch, &sh);
+      C64 c{ch, 0.0};
+      C64 s{0.0, -sh};
       m[0][0] = c; m[0][1] = s; m[1][0] = s; m[1][1] = c;
       return true;
     }
     case G_RY: {  // :120-128
-      double half = g.theta / 2.0;
-      double ch = std::cos(half), sh = std::sin(half);
+      double half = g.theta / 2.0, ch, sh;
+      cos_sin(half, &ch, &sh);
       m[0][0] = C64{ch, 0.0}; m[0][1] = C64{-sh, 0.0};
       m[1][0] = C64{sh, 0.0}; m[1][1] = C64{ch, 0.0};
       return true;

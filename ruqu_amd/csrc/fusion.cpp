@@ -31,12 +31,12 @@ struct Item {
   int mat;        // pool offset
   uint64_t touched() const {
     uint64_t m = 1ull << q0;
-    if (kind == FOP_U2 || kind == FOP_D2) m |= 1ull << q1;
+    if (kind == FOP_U2 || kind == FOP_D2 || kind == FOP_CZ) m |= 1ull << q1;
     if (ctl >= 0) m |= 1ull << ctl;
     return m;
   }
   uint64_t need() const {  // qubits that must be register qubits
-    if (kind == FOP_U1) return 1ull << q0;
+    if (kind == FOP_U1 || kind == FOP_CX) return 1ull << q0;
     if (kind == FOP_U2) return (1ull << q0) | (1ull << q1);
     return 0;
   }
@@ -107,13 +107,12 @@ int plan_fused(uint32_t nq, const rq_gate* gates, size_t ngates, const FusionCon
     matrix_2q(g, m);
     int off = (int)pool.size();
     switch (g.kind) {
-      case RQ_GATE_CNOT: {
-        rq_c64 x[4] = {{0, 0}, {1, 0}, {1, 0}, {0, 0}};
-        pool.insert(pool.end(), x, x + 4);
-        items.push_back(Item{FOP_U1, (int)g.q1, -1, (int)g.q0, off});
+      case RQ_GATE_CNOT:
+        items.push_back(Item{FOP_CX, (int)g.q1, -1, (int)g.q0, 0});
         break;
-      }
       case RQ_GATE_CZ:
+        items.push_back(Item{FOP_CZ, (int)g.q0, (int)g.q1, -1, 0});
+        break;
       case RQ_GATE_RZZ: {
         rq_c64 d[4] = {m[0], m[5], m[10], m[15]};
         pool.insert(pool.end(), d, d + 4);
@@ -245,7 +244,7 @@ int plan_fused(uint32_t nq, const rq_gate* gates, size_t ngates, const FusionCon
         FOp op{};
         op.kind = it.kind;
         op.mat = it.mat;
-        if (it.kind == FOP_U1) {
+        if (it.kind == FOP_U1 || it.kind == FOP_CX) {
           op.t0 = reg_of[it.q0];
           if (it.ctl >= 0) {
             int8_t loc; int32_t pos;
@@ -259,7 +258,7 @@ int plan_fused(uint32_t nq, const rq_gate* gates, size_t ngates, const FusionCon
           op.t1 = reg_of[it.q1];
         } else if (it.kind == FOP_D1) {
           locate(it.q0, &op.dloc0, &op.dpos0);
-        } else {
+        } else {  // D2, CZ
           locate(it.q0, &op.dloc0, &op.dpos0);
           locate(it.q1, &op.dloc1, &op.dpos1);
         }
@@ -403,6 +402,19 @@ void emulate_plan(const FusedPlan& plan, uint32_t nq, HC<T>* a) {
                 v[k] = hadd(hmul(mp[0], x), hmul(mp[1], y));
                 v[k | bb] = hadd(hmul(mp[2], x), hmul(mp[3], y));
               }
+            } else if (op.kind == FOP_CX) {
+              if ((tid & op.ctl_thr) != op.ctl_thr || (outer & op.ctl_out) != op.ctl_out) continue;
+              int bb = 1 << op.t0;
+              for (int k = 0; k < FUSE_R; ++k) {
+                if ((k & bb) || (k & (int)op.ctl_reg) != (int)op.ctl_reg) continue;
+                HC<T> x = v[k];
+                v[k] = v[k | bb];
+                v[k | bb] = x;
+              }
+            } else if (op.kind == FOP_CZ) {
+              for (int k = 0; k < FUSE_R; ++k)
+                if (host_loc_bit(op.dloc0, op.dpos0, k, tid, outer) & host_loc_bit(op.dloc1, op.dpos1, k, tid, outer))
+                  v[k] = HC<T>{-v[k].re, -v[k].im};
             } else if (op.kind == FOP_U2) {
               int b0 = 1 << op.t0, b1 = 1 << op.t1;
               for (int k = 0; k < FUSE_R; ++k) {

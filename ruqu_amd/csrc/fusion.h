@@ -25,7 +25,10 @@ constexpr int FUSE_R = 1 << FUSE_R_BITS;
 constexpr int FUSE_MAX_M = 12;         // largest tile (qubits): 256 threads x 16 amplitudes
 constexpr int FUSE_MAX_OUTER = 48;
 
-enum FOpKind : int32_t { FOP_U1 = 0, FOP_U2 = 1, FOP_D1 = 2, FOP_D2 = 3 };
+// U1: 2x2 on a register qubit (optionally controlled); U2: 4x4 on two register
+// qubits; D1/D2: diagonal factor read from one/two index bits; CX: controlled
+// swap of a register-qubit pair (CNOT); CZ: sign flip where both bits are 1.
+enum FOpKind : int32_t { FOP_U1 = 0, FOP_U2 = 1, FOP_D1 = 2, FOP_D2 = 3, FOP_CX = 4, FOP_CZ = 5 };
 
 // Where an index bit of an amplitude comes from inside a phase.
 enum BitLoc : int8_t { LOC_REG = 0, LOC_THR = 1, LOC_OUT = 2 };

@@ -1,13 +1,23 @@
 // gate.rs closed forms, evaluated on the host (see gates.h).
 #include "gates.h"
 
+#include <math.h>
+
 #include <cmath>
 
 namespace rq {
 
 namespace {
 constexpr double FRAC_1_SQRT_2 = 0.70710678118654752440084436210484903928483593768847;
-inline rq_c64 polar1(double th) { return rq_c64{1.0 * std::cos(th), 1.0 * std::sin(th)}; }  // types.rs:24-29
+// cos/sin pairs of one argument go through one glibc sincos call: the
+// reference's f64::cos/sin pair lowers to an LLVM `sincos` libcall on GNU
+// targets, and glibc's sincos can differ from separate sin/cos in the last bit.
+inline void cos_sin(double th, double* c, double* s) { ::sincos(th, s, c); }
+inline rq_c64 polar1(double th) {  // types.rs:24-29 from_polar(1.0, th)
+  double c, s;
+  cos_sin(th, &c, &s);
+  return rq_c64{1.0 * c, 1.0 * s};
+}
 }  // namespace
 
 bool matrix_1q(const rq_gate& g, rq_c64 m[4]) {
@@ -26,13 +36,15 @@ bool matrix_1q(const rq_gate& g, rq_c64 m[4]) {
     case RQ_GATE_T: m[0] = c1; m[1] = c0; m[2] = c0; m[3] = rq_c64{FRAC_1_SQRT_2, FRAC_1_SQRT_2}; return true;
     case RQ_GATE_TDG: m[0] = c1; m[1] = c0; m[2] = c0; m[3] = rq_c64{FRAC_1_SQRT_2, -FRAC_1_SQRT_2}; return true;
     case RQ_GATE_RX: {
-      double half = g.theta / 2.0;
-      rq_c64 c{std::cos(half), 0.0}, s{0.0, -std::sin(half)};
+      double half = g.theta / 2.0, ch, sh;
+      cos_sin(half, &ch, &sh);
+      rq_c64 c{ch, 0.0}, s{0.0, -sh};
       m[0] = c; m[1] = s; m[2] = s; m[3] = c;
       return true;
     }
     case RQ_GATE_RY: {
-      double half = g.theta / 2.0, ch = std::cos(half), sh = std::sin(half);
+      double half = g.theta / 2.0, ch, sh;
+      cos_sin(half, &ch, &sh);
       m[0] = rq_c64{ch, 0.0}; m[1] = rq_c64{-sh, 0.0}; m[2] = rq_c64{sh, 0.0}; m[3] = rq_c64{ch, 0.0};
       return true;
     }

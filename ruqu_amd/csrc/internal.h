@@ -48,6 +48,13 @@ double gate_bytes(const State& s, int kind);
 // Ensure at least `doubles` of reduction scratch on the state.
 int ensure_red(State& s, size_t doubles);
 
+// Shape knobs of the streaming (one pass per gate) kernels.
+struct StreamTune {
+  int unroll = 1;          // amplitude groups per thread: 1, 2, 4 or 8
+  bool nontemporal = true; // measured best on MI355X (30-qubit c32 sweep)
+};
+StreamTune& stream_tune();
+
 // ---- kernel launchers (kernels_gates.hip / kernels_reduce.hip) ----------
 // m: row-major 2x2 / 4x4 complex-f64 matrices; converted to the state's
 // precision on the host exactly as mixed_precision.rs:556-591 does.

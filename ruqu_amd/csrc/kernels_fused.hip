@@ -113,6 +113,30 @@ __device__ __forceinline__ void u2(Cx<T> (&v)[FUSE_R], const Cx<T>* __restrict__
   u2_all<B0, B1>(v, mp, std::make_integer_sequence<int, FUSE_R>{});
 }
 
+// Controlled swap on register bit B (CNOT target).
+template <int B, class T>
+__device__ __forceinline__ void cx(Cx<T> (&v)[FUSE_R], uint32_t ctl_reg) {
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) {
+    if (k & (1 << B)) continue;
+    const int k1 = k | (1 << B);
+    if ((k & ctl_reg) != ctl_reg) continue;
+    const Cx<T> x = v[k];
+    v[k] = v[k1];
+    v[k1] = x;
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void dispatch_cx(Cx<T> (&v)[FUSE_R], int b, uint32_t ctl_reg) {
+  switch (b) {
+    case 0: cx<0>(v, ctl_reg); break;
+    case 1: cx<1>(v, ctl_reg); break;
+    case 2: cx<2>(v, ctl_reg); break;
+    default: cx<3>(v, ctl_reg); break;
+  }
+}
+
 template <class T>
 __device__ __forceinline__ void dispatch_u1(Cx<T> (&v)[FUSE_R], int b, const Cx<T>* mp, uint32_t ctl_reg) {
   switch (b) {
@@ -162,11 +186,20 @@ __device__ __forceinline__ void apply_ops(Cx<T> (&v)[FUSE_R], const FOp* __restr
       case FOP_U2:
         dispatch_u2(v, op.t0, op.t1, mp);
         break;
+      case FOP_CX:
+        if ((tid & op.ctl_thr) == op.ctl_thr && (outer & op.ctl_out) == op.ctl_out) dispatch_cx(v, op.t0, op.ctl_reg);
+        break;
+      case FOP_CZ:
+#pragma unroll
+        for (int k = 0; k < FUSE_R; ++k)
+          if (loc_bit(op.dloc0, op.dpos0, k, tid, outer) & loc_bit(op.dloc1, op.dpos1, k, tid, outer))
+            v[k] = Cx<T>{-v[k].re, -v[k].im};
+        break;
       case FOP_D1:
 #pragma unroll
         for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(mp[loc_bit(op.dloc0, op.dpos0, k, tid, outer)], v[k]);
         break;
-      default:
+      default:  // FOP_D2
 #pragma unroll
         for (int k = 0; k < FUSE_R; ++k) {
           int i = loc_bit(op.dloc0, op.dpos0, k, tid, outer) * 2 + loc_bit(op.dloc1, op.dpos1, k, tid, outer);

@@ -42,6 +42,26 @@ struct M4 {
   Cx<T> m[16];
 };
 
+// HBM access with an optional non-temporal hint, through the 16-byte native
+// vector type (float4 and double2 are both 16 bytes).
+typedef float nvec4 __attribute__((ext_vector_type(4)));
+template <bool NT, class V>
+__device__ __forceinline__ V ldg(const V* p) {
+  static_assert(sizeof(V) == 16, "16-byte vectors only");
+  if constexpr (NT) {
+    nvec4 r = __builtin_nontemporal_load(reinterpret_cast<const nvec4*>(p));
+    return __builtin_bit_cast(V, r);
+  } else {
+    return *p;
+  }
+}
+template <bool NT, class V>
+__device__ __forceinline__ void stg(V* p, V v) {
+  static_assert(sizeof(V) == 16, "16-byte vectors only");
+  if constexpr (NT) __builtin_nontemporal_store(__builtin_bit_cast(nvec4, v), reinterpret_cast<nvec4*>(p));
+  else *p = v;
+}
+
 // Insert a zero bit at position q of p.
 __device__ __forceinline__ uint64_t ins0(uint64_t p, uint32_t q) {
   uint64_t lo = p & ((1ull << q) - 1);
@@ -49,7 +69,7 @@ __device__ __forceinline__ uint64_t ins0(uint64_t p, uint32_t q) {
 }
 
 // ---- 1q, f32, q >= 1: each group = two adjacent pairs (float4 each side) --
-template <int U>
+template <int U, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_1q_f32_v2(float4* __restrict__ a, uint32_t q, uint64_t ngroups,
                                                      M2<float> m) {
   const uint64_t g0 = (uint64_t)blockIdx.x * U * BLOCK + threadIdx.x;
@@ -62,8 +82,8 @@ __global__ __launch_bounds__(BLOCK) void k_1q_f32_v2(float4* __restrict__ a, uin
       uint64_t i = ins0(2 * g, q);
       ii[k] = i >> 1;
       jj[k] = (i + (1ull << q)) >> 1;
-      x[k] = a[ii[k]];
-      y[k] = a[jj[k]];
+      x[k] = ldg<NT>(a + ii[k]);
+      y[k] = ldg<NT>(a + jj[k]);
     }
   }
 #pragma unroll
@@ -76,21 +96,21 @@ __global__ __launch_bounds__(BLOCK) void k_1q_f32_v2(float4* __restrict__ a, uin
       Cx<float> o1 = cadd(cmul(m.m00, a1), cmul(m.m01, b1));
       Cx<float> p0 = cadd(cmul(m.m10, a0), cmul(m.m11, b0));
       Cx<float> p1 = cadd(cmul(m.m10, a1), cmul(m.m11, b1));
-      a[ii[k]] = make_float4(o0.re, o0.im, o1.re, o1.im);
-      a[jj[k]] = make_float4(p0.re, p0.im, p1.re, p1.im);
+      stg<NT>(a + ii[k], make_float4(o0.re, o0.im, o1.re, o1.im));
+      stg<NT>(a + jj[k], make_float4(p0.re, p0.im, p1.re, p1.im));
     }
   }
 }
 
 // ---- 1q, f32, q == 0: one float4 holds the pair (a, b) --------------------
-template <int U>
+template <int U, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_1q_f32_q0(float4* __restrict__ a, uint64_t npairs, M2<float> m) {
   const uint64_t g0 = (uint64_t)blockIdx.x * U * BLOCK + threadIdx.x;
   float4 x[U];
 #pragma unroll
   for (int k = 0; k < U; ++k) {
     uint64_t g = g0 + (uint64_t)k * BLOCK;
-    if (g < npairs) x[k] = a[g];
+    if (g < npairs) x[k] = ldg<NT>(a + g);
   }
 #pragma unroll
   for (int k = 0; k < U; ++k) {
@@ -99,13 +119,13 @@ __global__ __launch_bounds__(BLOCK) void k_1q_f32_q0(float4* __restrict__ a, uin
       Cx<float> av{x[k].x, x[k].y}, bv{x[k].z, x[k].w};
       Cx<float> o = cadd(cmul(m.m00, av), cmul(m.m01, bv));
       Cx<float> p = cadd(cmul(m.m10, av), cmul(m.m11, bv));
-      a[g] = make_float4(o.re, o.im, p.re, p.im);
+      stg<NT>(a + g, make_float4(o.re, o.im, p.re, p.im));
     }
   }
 }
 
 // ---- 1q, f64: one double2 per amplitude -----------------------------------
-template <int U>
+template <int U, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_1q_f64(double2* __restrict__ a, uint32_t q, uint64_t npairs,
                                                   M2<double> m) {
   const uint64_t g0 = (uint64_t)blockIdx.x * U * BLOCK + threadIdx.x;
@@ -116,8 +136,8 @@ __global__ __launch_bounds__(BLOCK) void k_1q_f64(double2* __restrict__ a, uint3
     uint64_t g = g0 + (uint64_t)k * BLOCK;
     if (g < npairs) {
       ii[k] = ins0(g, q);
-      x[k] = a[ii[k]];
-      y[k] = a[ii[k] + (1ull << q)];
+      x[k] = ldg<NT>(a + ii[k]);
+      y[k] = ldg<NT>(a + ii[k] + (1ull << q));
     }
   }
 #pragma unroll
@@ -127,8 +147,8 @@ __global__ __launch_bounds__(BLOCK) void k_1q_f64(double2* __restrict__ a, uint3
       Cx<double> av{x[k].x, x[k].y}, bv{y[k].x, y[k].y};
       Cx<double> o = cadd(cmul(m.m00, av), cmul(m.m01, bv));
       Cx<double> p = cadd(cmul(m.m10, av), cmul(m.m11, bv));
-      a[ii[k]] = make_double2(o.re, o.im);
-      a[ii[k] + (1ull << q)] = make_double2(p.re, p.im);
+      stg<NT>(a + ii[k], make_double2(o.re, o.im));
+      stg<NT>(a + ii[k] + (1ull << q), make_double2(p.re, p.im));
     }
   }
 }
@@ -262,7 +282,7 @@ __global__ __launch_bounds__(BLOCK) void k_scale(V* __restrict__ a, T f, uint64_
 }
 
 // ---- plain 16 B copy (bandwidth reference and state copies) ----------------
-template <int U>
+template <int U, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_copy16(float4* __restrict__ dst, const float4* __restrict__ src,
                                                   uint64_t nvec) {
   const uint64_t g0 = (uint64_t)blockIdx.x * U * BLOCK + threadIdx.x;
@@ -270,12 +290,12 @@ __global__ __launch_bounds__(BLOCK) void k_copy16(float4* __restrict__ dst, cons
 #pragma unroll
   for (int k = 0; k < U; ++k) {
     uint64_t g = g0 + (uint64_t)k * BLOCK;
-    if (g < nvec) x[k] = src[g];
+    if (g < nvec) x[k] = ldg<NT>(src + g);
   }
 #pragma unroll
   for (int k = 0; k < U; ++k) {
     uint64_t g = g0 + (uint64_t)k * BLOCK;
-    if (g < nvec) dst[g] = x[k];
+    if (g < nvec) stg<NT>(dst + g, x[k]);
   }
 }
 
@@ -297,9 +317,38 @@ M4<T> to_m4(const rq_c64* m) {
   return r;
 }
 
-constexpr int U1 = 4;  // pairs (or pair groups) per thread in the 1q kernels
+template <int U, bool NT>
+void launch_1q_t(State& s, uint32_t q, const rq_c64* m) {
+  if (s.prec == RQ_C64) {
+    uint64_t np = s.n / 2;
+    hipLaunchKernelGGL((k_1q_f64<U, NT>), grid_for(np, (uint64_t)BLOCK * U), dim3(BLOCK), 0, s.stream,
+                       (double2*)s.amps, q, np, to_m2<double>(m));
+  } else if (q == 0) {
+    uint64_t np = s.n / 2;
+    hipLaunchKernelGGL((k_1q_f32_q0<U, NT>), grid_for(np, (uint64_t)BLOCK * U), dim3(BLOCK), 0, s.stream,
+                       (float4*)s.amps, np, to_m2<float>(m));
+  } else {
+    uint64_t ng = s.n / 4;
+    hipLaunchKernelGGL((k_1q_f32_v2<U, NT>), grid_for(ng, (uint64_t)BLOCK * U), dim3(BLOCK), 0, s.stream,
+                       (float4*)s.amps, q, ng, to_m2<float>(m));
+  }
+}
+
+template <int U, bool NT>
+void launch_copy_t(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  uint64_t nv = bytes / 16;
+  hipLaunchKernelGGL((k_copy16<U, NT>), grid_for(nv, (uint64_t)BLOCK * U), dim3(BLOCK), 0, st, (float4*)dst,
+                     (const float4*)src, nv);
+}
 
 }  // namespace
+
+// Shape of the streaming kernels: amplitude groups per thread (1, 2, 4, 8)
+// and non-temporal loads/stores.  Chosen by measurement (rq_tune_set).
+StreamTune& stream_tune() {
+  static StreamTune t;
+  return t;
+}
 
 void launch_init_zero(State& s) {
   if (s.prec == RQ_C64)
@@ -309,18 +358,16 @@ void launch_init_zero(State& s) {
 }
 
 void launch_apply_1q(State& s, uint32_t q, const rq_c64* m) {
-  if (s.prec == RQ_C64) {
-    uint64_t np = s.n / 2;
-    hipLaunchKernelGGL(k_1q_f64<U1>, grid_for(np, (uint64_t)BLOCK * U1), dim3(BLOCK), 0, s.stream,
-                       (double2*)s.amps, q, np, to_m2<double>(m));
-  } else if (q == 0) {
-    uint64_t np = s.n / 2;
-    hipLaunchKernelGGL(k_1q_f32_q0<U1>, grid_for(np, (uint64_t)BLOCK * U1), dim3(BLOCK), 0, s.stream,
-                       (float4*)s.amps, np, to_m2<float>(m));
-  } else {
-    uint64_t ng = s.n / 4;
-    hipLaunchKernelGGL(k_1q_f32_v2<U1>, grid_for(ng, (uint64_t)BLOCK * U1), dim3(BLOCK), 0, s.stream,
-                       (float4*)s.amps, q, ng, to_m2<float>(m));
+  const StreamTune& t = stream_tune();
+  switch (t.unroll * 2 + (t.nontemporal ? 1 : 0)) {
+    case 2: launch_1q_t<1, false>(s, q, m); break;
+    case 3: launch_1q_t<1, true>(s, q, m); break;
+    case 4: launch_1q_t<2, false>(s, q, m); break;
+    case 5: launch_1q_t<2, true>(s, q, m); break;
+    case 9: launch_1q_t<4, true>(s, q, m); break;
+    case 16: launch_1q_t<8, false>(s, q, m); break;
+    case 17: launch_1q_t<8, true>(s, q, m); break;
+    default: launch_1q_t<4, false>(s, q, m); break;
   }
 }
 
@@ -390,9 +437,17 @@ void launch_scale(State& s, double f) {
 }
 
 void launch_plain_copy(void* dst, const void* src, size_t bytes, hipStream_t st) {
-  uint64_t nv = bytes / 16;
-  hipLaunchKernelGGL(k_copy16<4>, grid_for(nv, (uint64_t)BLOCK * 4), dim3(BLOCK), 0, st, (float4*)dst,
-                     (const float4*)src, nv);
+  const StreamTune& t = stream_tune();
+  switch (t.unroll * 2 + (t.nontemporal ? 1 : 0)) {
+    case 2: launch_copy_t<1, false>(dst, src, bytes, st); break;
+    case 3: launch_copy_t<1, true>(dst, src, bytes, st); break;
+    case 4: launch_copy_t<2, false>(dst, src, bytes, st); break;
+    case 5: launch_copy_t<2, true>(dst, src, bytes, st); break;
+    case 9: launch_copy_t<4, true>(dst, src, bytes, st); break;
+    case 16: launch_copy_t<8, false>(dst, src, bytes, st); break;
+    case 17: launch_copy_t<8, true>(dst, src, bytes, st); break;
+    default: launch_copy_t<4, false>(dst, src, bytes, st); break;
+  }
 }
 
 void launch_copy(State& dst, const State& src) { launch_plain_copy(dst.amps, src.amps, src.bytes(), dst.stream); }

@@ -701,3 +701,42 @@ int rq_time_copy(rq_state* h, int reps, double* avg_ms) {
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Tuning knobs (no reference counterpart).  key 0: streaming unroll (1, 2, 4,
+// 8); key 1: non-temporal loads/stores (0/1); key 2: fused tile qubits m;
+// key 3: fused low qubits c.
+int rq_tune_set(int key, int value) {
+  switch (key) {
+    case 0:
+      if (value != 1 && value != 2 && value != 4 && value != 8) return fail(RQ_ERR_ARGUMENT, "unroll must be 1, 2, 4 or 8");
+      stream_tune().unroll = value;
+      return RQ_OK;
+    case 1:
+      stream_tune().nontemporal = value != 0;
+      return RQ_OK;
+    case 2:
+      if (value < 4 || value > FUSE_MAX_M) return fail(RQ_ERR_ARGUMENT, "tile qubits out of range");
+      fusion_config().m = value;
+      return RQ_OK;
+    case 3:
+      if (value < 0 || value > FUSE_MAX_M) return fail(RQ_ERR_ARGUMENT, "low qubits out of range");
+      fusion_config().c = value;
+      return RQ_OK;
+    default:
+      return fail(RQ_ERR_ARGUMENT, "unknown tuning key");
+  }
+}
+
+int rq_tune_get(int key) {
+  switch (key) {
+    case 0: return stream_tune().unroll;
+    case 1: return stream_tune().nontemporal ? 1 : 0;
+    case 2: return fusion_config().m;
+    case 3: return fusion_config().c;
+    default: return -1;
+  }
+}
+
+}  // extern "C"
