@@ -224,6 +224,48 @@ int rq_argmax_probability(const rq_state* s, uint64_t* index, double* prob);
 /* <prod_{q in masks[k]} Z_q> for each mask, one FP64 pass per 32 masks. */
 int rq_expectation_zmasks(const rq_state* s, const uint64_t* masks, size_t n, double* out);
 
+/* ---- batched QAOA / VQE (configs with thousands of small instances) ---- */
+/* Many independent instances (2..24 qubits QAOA, 2..16 qubits VQE with the
+ * Ry-CZ ansatz) in complex-f32 with FP64 reductions, evaluated by one
+ * persistent kernel per optimizer round; optimizer steps follow run_qaoa
+ * (qaoa.rs:217-308) and run_vqe (vqe.rs:148-216) per instance. */
+typedef struct rq_batch rq_batch;
+int rq_batch_create(rq_batch** out);
+void rq_batch_destroy(rq_batch* b);
+/* init_* may be null: 0.5 each (qaoa.rs:221-222). */
+int rq_batch_add_qaoa(rq_batch* b, const rq_graph* g, uint32_t p, const double* init_gammas,
+                      const double* init_betas, uint32_t* id);
+/* Ry-CZ ansatz: per layer Ry(params[l*n+q]) on every qubit, then CZ(q, q+1);
+ * Hamiltonian term k = coeffs[k] * ops[offsets[k] .. offsets[k+1]).
+ * init_params may be null: 0.1 each (vqe.rs:152). */
+int rq_batch_add_vqe(rq_batch* b, uint32_t n, uint32_t layers, const double* coeffs, const uint64_t* offsets,
+                     const rq_pauli_factor* ops, size_t nterms, const double* init_params, uint32_t* id);
+/* Energy (and QAOA argmax index) of every instance at its current parameters. */
+int rq_batch_evaluate(rq_batch* b, double* energies, uint64_t* argmax);
+/* max_iter optimizer iterations for every instance; use_graph = 1 captures one
+ * iteration as a hipGraph and replays it. */
+int rq_batch_optimize(rq_batch* b, uint32_t max_iter, double qaoa_lr, double vqe_lr, double vqe_threshold,
+                      int use_graph, double* device_ms);
+/* Results of one instance: best energy (QAOA max / VQE min), QAOA best
+ * bitstring index, history length, converged flag, final parameters, VQE best
+ * parameters, energy history. */
+int rq_batch_result(rq_batch* b, uint32_t id, double* best, uint64_t* best_index, uint32_t* iterations,
+                    int* converged, double* params, double* best_params, double* history);
+int rq_batch_set_params(rq_batch* b, uint32_t id, const double* params);
+uint32_t rq_batch_size(const rq_batch* b);
+/* HIP-event time of the last evaluate/optimize call. */
+double rq_batch_last_ms(const rq_batch* b);
+/* Host-only inspection (no device call; no reference counterpart): the
+ * records the evaluation kernel receives.  insts: ninst 48-byte instance
+ * records; base_items: nbase 16-byte work items in launch order (largest
+ * first); slot_amps: amplitudes per HBM state slot; ctab_total: floats of the
+ * cut tables.  Null outputs are skipped (call once for the counts). */
+int rq_batch_plan_export(rq_batch* b, void* insts, void* base_items, int* ninst, int* nbase, uint64_t* slot_amps,
+                         int64_t* ctab_total);
+/* Host-only: edge (16 B), Pauli-term (32 B) and parameter records. */
+int rq_batch_data_export(rq_batch* b, void* edges, void* terms, double* params, size_t* nedges, size_t* nterms,
+                         size_t* nparams);
+
 /* ---- fused schedule (no reference counterpart: the reference's optimizer
  * only folds adjacent 1q gates, optimizer.rs:46-94, and is not called by
  * Simulator) ------------------------------------------------------------ */

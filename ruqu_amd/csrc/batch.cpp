@@ -1,0 +1,570 @@
+// Batched QAOA / VQE driver: instance registry, per-round work lists, the
+// device-side optimizer updates and the hipGraph of one optimizer iteration.
+//
+// Optimizer semantics per instance are those of the reference:
+//   QAOA plain (qaoa.rs:217-308): evaluate; history; best (> , argmax
+//   bitstring); converged when |cost - prev| < 1e-6 (iteration > 0); then
+//   gamma_0..gamma_{p-1} and beta_0..beta_{p-1} updated one after another,
+//   each from two pi/2-shifted evaluations at the angles updated so far.
+//   VQE (vqe.rs:148-216): evaluate; history; best (<, params); converged when
+//   |prev - e| < threshold; gradient of every parameter from two shifted
+//   evaluations at the same parameters; simultaneous descent update.
+// Round 0 of an iteration evaluates every QAOA base circuit and every VQE
+// circuit (base and all shifts); round r >= 1 evaluates the two shifted
+// circuits of QAOA parameter r-1.  Each round is one launch of the persistent
+// kernel; the update kernels run on the device between rounds, so a whole
+// iteration is captured once as a hipGraph and replayed.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "batch.h"
+#include "internal.h"
+
+namespace rq {
+
+__global__ void k_batch_eval(const BInst* insts, const BItem* items, int nitems, unsigned* counter,
+                             const int32_t* done, const BEdge* edges, const BTerm* terms, const double* params,
+                             const float* ctab, float2* slots, uint64_t slot_amps, double* energy, uint32_t* amax);
+__global__ void k_cut_table(float* ctab, int n, const BEdge* e, int ne);
+__global__ void k_zero_u32(unsigned* p, int n);
+
+namespace {
+
+struct OptState {
+  int32_t* iter;
+  int32_t* hlen;
+  int32_t* conv;
+  int32_t* done;
+  double* hist;
+  double* best;
+  uint32_t* best_idx;
+  double* best_params;
+};
+
+__global__ void k_opt_init(const BInst* insts, int ninst, OptState o) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ninst) return;
+  o.iter[k] = 0;
+  o.hlen[k] = 0;
+  o.conv[k] = 0;
+  o.done[k] = 0;
+  o.best[k] = insts[k].kind == BK_QAOA ? -INFINITY : 1.7976931348623157e308;
+  o.best_idx[k] = 0;
+}
+
+__global__ void k_update0(const BInst* insts, int ninst, const int32_t* out0, const double* energy,
+                          const uint32_t* amax, double* params, OptState o, int max_iter, double vqe_lr,
+                          double vqe_thr) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ninst || o.done[k]) return;
+  const BInst in = insts[k];
+  const int it = o.iter[k];
+  const double e = energy[out0[k]];
+  double* h = o.hist + (size_t)k * max_iter;
+  h[it] = e;
+  o.hlen[k] = it + 1;
+  if (in.kind == BK_QAOA) {
+    if (e > o.best[k]) {
+      o.best[k] = e;
+      o.best_idx[k] = amax[out0[k]];
+    }
+    if (it > 0 && fabs(e - h[it - 1]) < 1e-6) {
+      o.conv[k] = 1;
+      o.done[k] = 1;
+    }
+  } else {
+    if (e < o.best[k]) {
+      o.best[k] = e;
+      for (int j = 0; j < in.nparams; ++j) o.best_params[in.param_off + j] = params[in.param_off + j];
+    }
+    if (it > 0 && fabs(h[it - 1] - e) < vqe_thr) {
+      o.conv[k] = 1;
+      o.done[k] = 1;
+      return;
+    }
+    for (int j = 0; j < in.nparams; ++j) {
+      double g = (energy[out0[k] + 1 + 2 * j] - energy[out0[k] + 2 + 2 * j]) / 2.0;
+      params[in.param_off + j] -= vqe_lr * g;
+    }
+    o.iter[k] = it + 1;
+    if (it + 1 >= max_iter) o.done[k] = 1;
+  }
+}
+
+__global__ void k_update_r(const BInst* insts, int ninst, int r, const int32_t* outr, const double* energy,
+                           double* params, OptState o, double qaoa_lr) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ninst || o.done[k]) return;
+  const BInst in = insts[k];
+  if (in.kind != BK_QAOA || r > in.nparams) return;
+  const int o_ = outr[k] + 2 * (r - 1);
+  const double cp = energy[o_], cm = energy[o_ + 1];
+  params[in.param_off + r - 1] += qaoa_lr * (cp - cm) / 2.0;  // qaoa.rs:285,296
+}
+
+__global__ void k_end_iter(const BInst* insts, int ninst, OptState o, int max_iter) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ninst || o.done[k] || insts[k].kind != BK_QAOA) return;
+  o.iter[k] += 1;
+  if (o.iter[k] >= max_iter) o.done[k] = 1;
+}
+
+template <class T>
+int upload(T** d, const std::vector<T>& h) {
+  size_t bytes = std::max<size_t>(h.size(), 1) * sizeof(T);
+  RQ_HIP(hipMalloc((void**)d, bytes));
+  if (!h.empty()) RQ_HIP(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return RQ_OK;
+}
+
+}  // namespace
+
+struct Batch {
+  std::vector<BInst> insts;
+  std::vector<BEdge> edges;
+  std::vector<BTerm> terms;
+  std::vector<double> params;
+  bool ready = false;
+  int nwg = 0;
+  hipStream_t st = nullptr;
+  BInst* d_insts = nullptr;
+  BEdge* d_edges = nullptr;
+  BTerm* d_terms = nullptr;
+  double* d_params = nullptr;
+  float* d_ctab = nullptr;
+  float2* d_slots = nullptr;
+  uint64_t slot_amps = 0;
+  int nrounds = 0;
+  std::vector<int> round_off, round_n;
+  BItem* d_items = nullptr;
+  BItem* d_base_items = nullptr;
+  int nbase = 0;
+  int nouts = 0;
+  double* d_energy = nullptr;
+  uint32_t* d_amax = nullptr;
+  unsigned* d_counters = nullptr;
+  int32_t* d_out0 = nullptr;
+  int32_t* d_outr = nullptr;
+  std::vector<int32_t> out0, outr;
+  OptState o{};
+  int hist_cap = 0;
+  bool planned = false;
+  int64_t ctab_total = 0;
+  std::vector<BItem> all_items, base_items;
+  double* d_best_params = nullptr;
+  int32_t* d_zero_done = nullptr;  // all-zero flags for plain evaluation
+  hipGraphExec_t exec = nullptr;
+  int graph_iter_cap = -1;
+  double last_ms = 0;
+};
+
+namespace {
+
+double item_cost(const BInst& in) {
+  double amps = std::ldexp(1.0, in.n);
+  return in.kind == BK_QAOA ? amps * (in.p + 1) : amps * (in.p + in.nterms);
+}
+
+// Host part of the setup: cut-table offsets, slot size, the per-round work
+// lists and output slots.  No device call.
+void build_plan(Batch& b) {
+  const int ninst = (int)b.insts.size();
+  b.ctab_total = 0;
+  int maxq_hbm = 0;
+  for (BInst& in : b.insts) {
+    if (in.n > BATCH_LDS_QUBITS) maxq_hbm = std::max(maxq_hbm, in.n);
+    if (in.kind == BK_QAOA && in.n > BATCH_LDS_QUBITS) {
+      in.ctab_off = b.ctab_total;
+      b.ctab_total += (int64_t)1 << in.n;
+    } else {
+      in.ctab_off = -1;
+    }
+  }
+  b.slot_amps = maxq_hbm ? (1ull << maxq_hbm) : 0;
+  int pmax2 = 0;
+  for (const BInst& in : b.insts)
+    if (in.kind == BK_QAOA) pmax2 = std::max(pmax2, in.nparams);
+  b.nrounds = 1 + pmax2;
+  std::vector<std::vector<BItem>> rounds(b.nrounds);
+  b.out0.assign(ninst, -1);
+  b.outr.assign(ninst, -1);
+  int out = 0;
+  b.base_items.clear();
+  for (int k = 0; k < ninst; ++k) {
+    const BInst& in = b.insts[k];
+    b.out0[k] = out;
+    rounds[0].push_back(BItem{k, 0, -1, out++});
+    b.base_items.push_back(BItem{k, 0, -1, b.out0[k]});
+    if (in.kind == BK_VQE)
+      for (int j = 0; j < in.nparams; ++j) {
+        rounds[0].push_back(BItem{k, 1, j, out++});
+        rounds[0].push_back(BItem{k, 2, j, out++});
+      }
+  }
+  for (int k = 0; k < ninst; ++k) {
+    const BInst& in = b.insts[k];
+    if (in.kind != BK_QAOA) continue;
+    b.outr[k] = out;
+    for (int j = 0; j < in.nparams; ++j) {
+      rounds[1 + j].push_back(BItem{k, 1, j, out++});
+      rounds[1 + j].push_back(BItem{k, 2, j, out++});
+    }
+  }
+  b.nouts = out;
+  b.all_items.clear();
+  b.round_off.clear();
+  b.round_n.clear();
+  for (int r = 0; r < b.nrounds; ++r) {
+    auto& v = rounds[r];
+    std::stable_sort(v.begin(), v.end(), [&](const BItem& x, const BItem& y) {
+      return item_cost(b.insts[x.inst]) > item_cost(b.insts[y.inst]);
+    });
+    b.round_off.push_back((int)b.all_items.size());
+    b.round_n.push_back((int)v.size());
+    b.all_items.insert(b.all_items.end(), v.begin(), v.end());
+  }
+  std::stable_sort(b.base_items.begin(), b.base_items.end(), [&](const BItem& x, const BItem& y) {
+    return item_cost(b.insts[x.inst]) > item_cost(b.insts[y.inst]);
+  });
+  b.nbase = (int)b.base_items.size();
+  b.planned = true;
+}
+
+int finalize(Batch& b) {
+  if (b.ready) return RQ_OK;
+  if (b.insts.empty()) return fail(RQ_ERR_ARGUMENT, "empty batch");
+  if (!b.planned) build_plan(b);
+  int dev = 0, cus = 0;
+  RQ_HIP(hipGetDevice(&dev));
+  RQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  b.nwg = 2 * std::max(cus, 1);
+  RQ_HIP(hipStreamCreateWithFlags(&b.st, hipStreamNonBlocking));
+  RQ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_eval), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (1 << BATCH_LDS_QUBITS) * 8));
+  const int ninst = (int)b.insts.size();
+  int rc = upload(&b.d_insts, b.insts);
+  if (!rc) rc = upload(&b.d_edges, b.edges);
+  if (!rc) rc = upload(&b.d_terms, b.terms);
+  if (!rc) rc = upload(&b.d_params, b.params);
+  if (rc) return rc;
+  RQ_HIP(hipMalloc((void**)&b.d_best_params, std::max<size_t>(b.params.size(), 1) * sizeof(double)));
+  RQ_HIP(hipMalloc((void**)&b.d_ctab, std::max<int64_t>(b.ctab_total, 1) * sizeof(float)));
+  for (const BInst& in : b.insts)
+    if (in.ctab_off >= 0)
+      hipLaunchKernelGGL(k_cut_table, dim3(1024), dim3(BATCH_THREADS), 0, b.st, b.d_ctab + in.ctab_off, in.n,
+                         b.d_edges + in.term_off, in.nterms);
+  if (b.slot_amps) {
+    hipError_t e = hipMalloc((void**)&b.d_slots, (size_t)b.nwg * b.slot_amps * sizeof(float2));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(RQ_ERR_ALLOC, "batch state slots: need " + std::to_string((size_t)b.nwg * b.slot_amps * 8) + " bytes");
+    }
+  }
+  rc = upload(&b.d_items, b.all_items);
+  if (!rc) rc = upload(&b.d_base_items, b.base_items);
+  if (!rc) rc = upload(&b.d_out0, b.out0);
+  if (!rc) rc = upload(&b.d_outr, b.outr);
+  if (rc) return rc;
+  RQ_HIP(hipMalloc((void**)&b.d_energy, std::max(b.nouts, 1) * sizeof(double)));
+  RQ_HIP(hipMalloc((void**)&b.d_amax, std::max(b.nouts, 1) * sizeof(uint32_t)));
+  RQ_HIP(hipMalloc((void**)&b.d_counters, (b.nrounds + 1) * sizeof(unsigned)));
+  std::vector<int32_t> zeros(ninst, 0);
+  rc = upload(&b.d_zero_done, zeros);
+  if (rc) return rc;
+  RQ_HIP(hipMalloc((void**)&b.o.iter, ninst * sizeof(int32_t)));
+  RQ_HIP(hipMalloc((void**)&b.o.hlen, ninst * sizeof(int32_t)));
+  RQ_HIP(hipMalloc((void**)&b.o.conv, ninst * sizeof(int32_t)));
+  RQ_HIP(hipMalloc((void**)&b.o.done, ninst * sizeof(int32_t)));
+  RQ_HIP(hipMalloc((void**)&b.o.best, ninst * sizeof(double)));
+  RQ_HIP(hipMalloc((void**)&b.o.best_idx, ninst * sizeof(uint32_t)));
+  b.o.best_params = b.d_best_params;
+  RQ_HIP(hipStreamSynchronize(b.st));
+  b.ready = true;
+  return RQ_OK;
+}
+
+void launch_eval(Batch& b, const BItem* items, int n, unsigned* counter, const int32_t* done) {
+  const unsigned grid = (unsigned)std::min(b.nwg, std::max(n, 1));
+  hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BATCH_THREADS), (1 << BATCH_LDS_QUBITS) * 8, b.st, b.d_insts,
+                     items, n, counter, done, b.d_edges, b.d_terms, b.d_params, b.d_ctab, b.d_slots, b.slot_amps,
+                     b.d_energy, b.d_amax);
+}
+
+void enqueue_iteration(Batch& b, int max_iter, double qaoa_lr, double vqe_lr, double vqe_thr) {
+  const int ninst = (int)b.insts.size();
+  const dim3 g1((ninst + 255) / 256), t1(256);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, b.st, b.d_counters, b.nrounds + 1);
+  for (int r = 0; r < b.nrounds; ++r) {
+    launch_eval(b, b.d_items + b.round_off[r], b.round_n[r], b.d_counters + r, b.o.done);
+    if (r == 0)
+      hipLaunchKernelGGL(k_update0, g1, t1, 0, b.st, b.d_insts, ninst, b.d_out0, b.d_energy, b.d_amax, b.d_params,
+                         b.o, max_iter, vqe_lr, vqe_thr);
+    else
+      hipLaunchKernelGGL(k_update_r, g1, t1, 0, b.st, b.d_insts, ninst, r, b.d_outr, b.d_energy, b.d_params, b.o,
+                         qaoa_lr);
+  }
+  hipLaunchKernelGGL(k_end_iter, g1, t1, 0, b.st, b.d_insts, ninst, b.o, max_iter);
+}
+
+void free_batch(Batch& b) {
+  if (b.st) (void)hipStreamSynchronize(b.st);
+  if (b.exec) (void)hipGraphExecDestroy(b.exec);
+  void* ptrs[] = {b.d_insts, b.d_edges, b.d_terms, b.d_params, b.d_ctab, b.d_slots, b.d_items, b.d_base_items,
+                  b.d_energy, b.d_amax, b.d_counters, b.d_out0, b.d_outr, b.o.iter, b.o.hlen, b.o.conv, b.o.done,
+                  b.o.hist, b.o.best, b.o.best_idx, b.d_best_params, b.d_zero_done};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (b.st) (void)hipStreamDestroy(b.st);
+}
+
+}  // namespace
+}  // namespace rq
+
+using namespace rq;
+
+extern "C" {
+
+int rq_batch_create(rq_batch** out) {
+  if (!out) return fail(RQ_ERR_ARGUMENT, "null pointer");
+  *out = (rq_batch*)new Batch();
+  return RQ_OK;
+}
+
+void rq_batch_destroy(rq_batch* h) {
+  Batch* b = (Batch*)h;
+  if (!b) return;
+  free_batch(*b);
+  delete b;
+}
+
+int rq_batch_add_qaoa(rq_batch* h, const rq_graph* g, uint32_t p, const double* init_gammas, const double* init_betas,
+                      uint32_t* id) {
+  Batch* b = (Batch*)h;
+  if (!b || !g || (g->num_edges && (!g->ei || !g->ej || !g->w))) return fail(RQ_ERR_ARGUMENT, "null pointer");
+  if (b->ready) return fail(RQ_ERR_ARGUMENT, "batch already finalized");
+  if (g->num_nodes < 2 || g->num_nodes > (uint32_t)BATCH_MAX_QUBITS)
+    return fail(RQ_ERR_QUBIT_LIMIT, "batched instances take 2.." + std::to_string(BATCH_MAX_QUBITS) + " qubits");
+  for (uint32_t e = 0; e < g->num_edges; ++e)
+    if (g->ei[e] >= g->num_nodes || g->ej[e] >= g->num_nodes)
+      return fail(RQ_ERR_INVALID_QUBIT, "node index out of range");
+  BInst in{};
+  in.kind = BK_QAOA;
+  in.n = (int)g->num_nodes;
+  in.p = (int)p;
+  in.nparams = 2 * (int)p;
+  in.nterms = (int)g->num_edges;
+  in.term_off = (int)b->edges.size();
+  in.param_off = (int)b->params.size();
+  in.wsum = 0.0;
+  for (uint32_t e = 0; e < g->num_edges; ++e) {
+    b->edges.push_back(BEdge{g->ei[e], g->ej[e], g->w[e]});
+    in.wsum += g->w[e];
+  }
+  for (uint32_t l = 0; l < p; ++l) b->params.push_back(init_gammas ? init_gammas[l] : 0.5);  // qaoa.rs:221
+  for (uint32_t l = 0; l < p; ++l) b->params.push_back(init_betas ? init_betas[l] : 0.5);    // qaoa.rs:222
+  if (id) *id = (uint32_t)b->insts.size();
+  b->insts.push_back(in);
+  return RQ_OK;
+}
+
+int rq_batch_add_vqe(rq_batch* h, uint32_t n, uint32_t layers, const double* coeffs, const uint64_t* offsets,
+                     const rq_pauli_factor* ops, size_t nterms, const double* init_params, uint32_t* id) {
+  Batch* b = (Batch*)h;
+  if (!b || (nterms && (!coeffs || !offsets))) return fail(RQ_ERR_ARGUMENT, "null pointer");
+  if (b->ready) return fail(RQ_ERR_ARGUMENT, "batch already finalized");
+  if (n < 2 || n > 16) return fail(RQ_ERR_QUBIT_LIMIT, "batched VQE instances take 2..16 qubits");
+  BInst in{};
+  in.kind = BK_VQE;
+  in.n = (int)n;
+  in.p = (int)layers;
+  in.nparams = (int)(n * layers);
+  in.nterms = (int)nterms;
+  in.term_off = (int)b->terms.size();
+  in.param_off = (int)b->params.size();
+  for (size_t k = 0; k < nterms; ++k) {
+    BTerm t{0, 0, 0, 0, coeffs[k]};
+    for (uint64_t f = offsets[k]; f < offsets[k + 1]; ++f) {  // canonical form as state.rs:370-392
+      uint32_t q = ops[f].qubit;
+      int op = ops[f].op;
+      if (op < 0 || op > 3) return fail(RQ_ERR_ARGUMENT, "unknown Pauli operator");
+      if (op == RQ_PAULI_I) continue;
+      if (q >= n) return fail(RQ_ERR_INVALID_QUBIT, "Pauli factor on a qubit beyond the instance");
+      if (op == RQ_PAULI_X || op == RQ_PAULI_Y) t.x ^= 1ull << q;
+      if (op == RQ_PAULI_Y || op == RQ_PAULI_Z) t.zy ^= 1ull << q;
+      if (op == RQ_PAULI_Y) t.ny += 1;
+    }
+    t.ny &= 3;
+    b->terms.push_back(t);
+  }
+  for (int j = 0; j < in.nparams; ++j) b->params.push_back(init_params ? init_params[j] : 0.1);  // vqe.rs:152
+  if (id) *id = (uint32_t)b->insts.size();
+  b->insts.push_back(in);
+  return RQ_OK;
+}
+
+// Energy of every instance at its current parameters (one persistent launch).
+int rq_batch_evaluate(rq_batch* h, double* energies, uint64_t* argmax) {
+  Batch* b = (Batch*)h;
+  if (!b || !energies) return fail(RQ_ERR_ARGUMENT, "null pointer");
+  int rc = finalize(*b);
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  RQ_HIP(hipEventCreate(&e0));
+  RQ_HIP(hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, b->st, b->d_counters, b->nrounds + 1);
+  RQ_HIP(hipEventRecord(e0, b->st));
+  launch_eval(*b, b->d_base_items, b->nbase, b->d_counters + b->nrounds, b->d_zero_done);
+  RQ_HIP(hipEventRecord(e1, b->st));
+  RQ_HIP(hipGetLastError());
+  RQ_HIP(hipStreamSynchronize(b->st));
+  float ms = 0;
+  RQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+  b->last_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::vector<double> en(b->nouts);
+  std::vector<uint32_t> am(b->nouts);
+  RQ_HIP(hipMemcpy(en.data(), b->d_energy, b->nouts * sizeof(double), hipMemcpyDeviceToHost));
+  RQ_HIP(hipMemcpy(am.data(), b->d_amax, b->nouts * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < b->insts.size(); ++k) {
+    energies[k] = en[b->out0[k]];
+    if (argmax) argmax[k] = am[b->out0[k]];
+  }
+  return RQ_OK;
+}
+
+// Runs max_iter optimizer iterations for every instance (use_graph: capture
+// one iteration as a hipGraph and replay it).
+int rq_batch_optimize(rq_batch* h, uint32_t max_iter, double qaoa_lr, double vqe_lr, double vqe_threshold,
+                      int use_graph, double* device_ms) {
+  Batch* b = (Batch*)h;
+  if (!b) return fail(RQ_ERR_ARGUMENT, "null batch");
+  int rc = finalize(*b);
+  if (rc) return rc;
+  const int ninst = (int)b->insts.size();
+  if ((int)max_iter > b->hist_cap) {
+    if (b->o.hist) RQ_HIP(hipFree(b->o.hist));
+    RQ_HIP(hipMalloc((void**)&b->o.hist, (size_t)ninst * std::max<uint32_t>(max_iter, 1) * sizeof(double)));
+    b->hist_cap = (int)max_iter;
+    if (b->exec) {
+      (void)hipGraphExecDestroy(b->exec);
+      b->exec = nullptr;
+    }
+  }
+  hipLaunchKernelGGL(k_opt_init, dim3((ninst + 255) / 256), dim3(256), 0, b->st, b->d_insts, ninst, b->o);
+  RQ_HIP(hipGetLastError());
+  hipEvent_t e0, e1;
+  RQ_HIP(hipEventCreate(&e0));
+  RQ_HIP(hipEventCreate(&e1));
+  if (use_graph && (!b->exec || b->graph_iter_cap != (int)max_iter)) {
+    if (b->exec) {
+      (void)hipGraphExecDestroy(b->exec);
+      b->exec = nullptr;
+    }
+    hipGraph_t graph;
+    RQ_HIP(hipStreamSynchronize(b->st));
+    RQ_HIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+    enqueue_iteration(*b, (int)max_iter, qaoa_lr, vqe_lr, vqe_threshold);
+    RQ_HIP(hipStreamEndCapture(b->st, &graph));
+    RQ_HIP(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0));
+    RQ_HIP(hipGraphDestroy(graph));
+    b->graph_iter_cap = (int)max_iter;
+  }
+  RQ_HIP(hipEventRecord(e0, b->st));
+  for (uint32_t it = 0; it < max_iter; ++it) {
+    if (use_graph) RQ_HIP(hipGraphLaunch(b->exec, b->st));
+    else enqueue_iteration(*b, (int)max_iter, qaoa_lr, vqe_lr, vqe_threshold);
+  }
+  RQ_HIP(hipEventRecord(e1, b->st));
+  RQ_HIP(hipGetLastError());
+  RQ_HIP(hipStreamSynchronize(b->st));
+  float ms = 0;
+  RQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+  b->last_ms = ms;
+  if (device_ms) *device_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return RQ_OK;
+}
+
+// Per-instance results after rq_batch_optimize.  params receives the final
+// parameters (QAOA: gammas then betas), best_params the best VQE parameters.
+int rq_batch_result(rq_batch* h, uint32_t id, double* best, uint64_t* best_index, uint32_t* iterations,
+                    int* converged, double* params, double* best_params, double* history) {
+  Batch* b = (Batch*)h;
+  if (!b || !b->ready || id >= b->insts.size()) return fail(RQ_ERR_ARGUMENT, "bad batch or instance id");
+  const BInst& in = b->insts[id];
+  int32_t hl = 0, cv = 0;
+  uint32_t bi = 0;
+  double bv = 0;
+  if (b->o.hist) {
+    RQ_HIP(hipMemcpy(&hl, b->o.hlen + id, sizeof(hl), hipMemcpyDeviceToHost));
+    RQ_HIP(hipMemcpy(&cv, b->o.conv + id, sizeof(cv), hipMemcpyDeviceToHost));
+    RQ_HIP(hipMemcpy(&bi, b->o.best_idx + id, sizeof(bi), hipMemcpyDeviceToHost));
+    RQ_HIP(hipMemcpy(&bv, b->o.best + id, sizeof(bv), hipMemcpyDeviceToHost));
+    if (history && hl > 0)
+      RQ_HIP(hipMemcpy(history, b->o.hist + (size_t)id * b->hist_cap, hl * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  if (best) *best = bv;
+  if (best_index) *best_index = bi;
+  if (iterations) *iterations = (uint32_t)hl;
+  if (converged) *converged = cv;
+  if (params)
+    RQ_HIP(hipMemcpy(params, b->d_params + in.param_off, in.nparams * sizeof(double), hipMemcpyDeviceToHost));
+  if (best_params && in.kind == BK_VQE)
+    RQ_HIP(hipMemcpy(best_params, b->d_best_params + in.param_off, in.nparams * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  return RQ_OK;
+}
+
+// Overwrites the parameters of one instance (e.g. a warm start) before the
+// next evaluate/optimize.
+int rq_batch_set_params(rq_batch* h, uint32_t id, const double* params) {
+  Batch* b = (Batch*)h;
+  if (!b || !params || id >= b->insts.size()) return fail(RQ_ERR_ARGUMENT, "bad batch or instance id");
+  const BInst& in = b->insts[id];
+  std::copy(params, params + in.nparams, b->params.begin() + in.param_off);
+  if (b->ready)
+    RQ_HIP(hipMemcpy(b->d_params + in.param_off, params, in.nparams * sizeof(double), hipMemcpyHostToDevice));
+  return RQ_OK;
+}
+
+// Host-only: the records the device kernels receive for the plain evaluation
+// (instances, the sorted base work list), for checks without a GPU.  Pass
+// null pointers to query the counts.
+int rq_batch_plan_export(rq_batch* h, void* insts, void* base_items, int* ninst, int* nbase, uint64_t* slot_amps,
+                         int64_t* ctab_total) {
+  Batch* b = (Batch*)h;
+  if (!b || b->insts.empty()) return fail(RQ_ERR_ARGUMENT, "empty batch");
+  if (!b->planned) build_plan(*b);
+  if (ninst) *ninst = (int)b->insts.size();
+  if (nbase) *nbase = b->nbase;
+  if (slot_amps) *slot_amps = b->slot_amps;
+  if (ctab_total) *ctab_total = b->ctab_total;
+  if (insts) std::memcpy(insts, b->insts.data(), b->insts.size() * sizeof(BInst));
+  if (base_items) std::memcpy(base_items, b->base_items.data(), b->base_items.size() * sizeof(BItem));
+  return RQ_OK;
+}
+
+// Host-only: the edge, term and parameter records in device layout.
+int rq_batch_data_export(rq_batch* h, void* edges, void* terms, double* params, size_t* nedges, size_t* nterms,
+                         size_t* nparams) {
+  Batch* b = (Batch*)h;
+  if (!b) return fail(RQ_ERR_ARGUMENT, "null batch");
+  if (nedges) *nedges = b->edges.size();
+  if (nterms) *nterms = b->terms.size();
+  if (nparams) *nparams = b->params.size();
+  if (edges) std::memcpy(edges, b->edges.data(), b->edges.size() * sizeof(BEdge));
+  if (terms) std::memcpy(terms, b->terms.data(), b->terms.size() * sizeof(BTerm));
+  if (params) std::memcpy(params, b->params.data(), b->params.size() * sizeof(double));
+  return RQ_OK;
+}
+
+uint32_t rq_batch_size(const rq_batch* h) { return h ? (uint32_t)((const Batch*)h)->insts.size() : 0; }
+
+double rq_batch_last_ms(const rq_batch* h) { return h ? ((const Batch*)h)->last_ms : 0.0; }
+
+}  // extern "C"

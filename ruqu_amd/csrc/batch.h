@@ -1,0 +1,50 @@
+// Batched QAOA / VQE: many small, uneven instances evaluated by one
+// persistent kernel (kernels_batch.hip), optimised by device-side update
+// kernels (batch.cpp), one optimizer iteration per hipGraph replay.
+#pragma once
+
+#include <cstdint>
+
+namespace rq {
+
+constexpr int BATCH_LDS_QUBITS = 13;       // states up to 2^13 c32 amplitudes live in LDS (64 KiB)
+constexpr int BATCH_TILE_QUBITS = 13;      // tile of the HBM-resident states
+constexpr int BATCH_MAX_QUBITS = 24;
+constexpr int BATCH_THREADS = 256;
+
+enum BatchKind : int32_t { BK_QAOA = 0, BK_VQE = 1 };
+
+struct BInst {
+  int32_t kind;
+  int32_t n;
+  int32_t p;          // QAOA depth or VQE layers
+  int32_t nparams;    // QAOA 2p (gammas then betas), VQE n*layers
+  int32_t nterms;     // QAOA edges or VQE Pauli terms
+  int32_t term_off;   // first edge / term
+  int32_t param_off;  // first parameter (doubles)
+  int32_t pad;
+  int64_t ctab_off;   // QAOA, n > BATCH_LDS_QUBITS: offset of the cut table (floats), else -1
+  double wsum;        // QAOA: total edge weight
+};
+
+struct BEdge {
+  uint32_t i, j;
+  double w;
+};
+
+struct BTerm {
+  uint64_t x, zy;
+  uint32_t ny, pad;
+  double c;
+};
+
+// One circuit evaluation: instance, variant (0 base, 1 +pi/2, 2 -pi/2 on
+// parameter `param`), and the slot of its energy in the output array.
+struct BItem {
+  int32_t inst;
+  int32_t variant;
+  int32_t param;
+  int32_t out;
+};
+
+}  // namespace rq

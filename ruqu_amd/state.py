@@ -269,6 +269,10 @@ class QuantumState:
         check(lib().rq_expectation_z_all(self._h, ptr(out)))
         return out
 
+    def set_zero(self):
+        """Back to |0...0> in place (the state QuantumState::new starts in)."""
+        check(lib().rq_state_set_zero(self._h))
+
     def normalize(self):
         check(lib().rq_normalize(self._h))
 
