@@ -255,16 +255,23 @@ int rq_batch_set_params(rq_batch* b, uint32_t id, const double* params);
 uint32_t rq_batch_size(const rq_batch* b);
 /* HIP-event time of the last evaluate/optimize call. */
 double rq_batch_last_ms(const rq_batch* b);
-/* Host-only inspection (no device call; no reference counterpart): the
- * records the evaluation kernel receives.  insts: ninst 48-byte instance
- * records; base_items: nbase 16-byte work items in launch order (largest
- * first); slot_amps: amplitudes per HBM state slot; ctab_total: floats of the
- * cut tables.  Null outputs are skipped (call once for the counts). */
-int rq_batch_plan_export(rq_batch* b, void* insts, void* base_items, int* ninst, int* nbase, uint64_t* slot_amps,
-                         int64_t* ctab_total);
-/* Host-only: edge (16 B), Pauli-term (32 B) and parameter records. */
-int rq_batch_data_export(rq_batch* b, void* edges, void* terms, double* params, size_t* nedges, size_t* nterms,
-                         size_t* nparams);
+/* LDS tile of the states above 13 qubits: 2^tq amplitudes per workgroup,
+ * ceil(n_max/2) <= tq <= 13 (no reference counterpart; the autotuner's knob). */
+int rq_batch_set_tile(rq_batch* b, int tq);
+int rq_batch_get_tile(const rq_batch* b);
+/* Host-only inspection (no device call; no reference counterpart): the launch
+ * plan of one round (round < 0: the plain evaluation; else round r of an
+ * optimizer iteration) as the device receives it.  info[8] = {nsmall, nwide,
+ * ntiles, nlayers, nrounds, nouts, tq, 0}; info64[2] = {amplitudes of the
+ * round's wide states, cut-table floats}; small/wide items are 16-byte
+ * records; wtile0/wstate: first tile and state offset per wide item; wmap:
+ * wide item of each tile; layer_grid: tiles launched at each layer.  Null
+ * outputs are skipped (call once for the counts). */
+int rq_batch_plan_export(rq_batch* b, int round, int32_t* info, int64_t* info64, void* small_items,
+                         void* wide_items, int32_t* wtile0, int64_t* wstate, int32_t* wmap, int32_t* layer_grid);
+/* Host-only: instance (48 B), edge (16 B), Pauli-term (32 B) and parameter
+ * records in device layout; counts[4] = {instances, edges, terms, params}. */
+int rq_batch_data_export(rq_batch* b, void* insts, void* edges, void* terms, double* params, size_t* counts);
 
 /* ---- fused schedule (no reference counterpart: the reference's optimizer
  * only folds adjacent 1q gates, optimizer.rs:46-94, and is not called by

@@ -29,6 +29,8 @@ _native.register("rq_batch_result", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(
 _native.register("rq_batch_set_params", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p])
 _native.register("rq_batch_size", C.c_uint32, [C.c_void_p])
 _native.register("rq_batch_last_ms", C.c_double, [C.c_void_p])
+_native.register("rq_batch_set_tile", C.c_int, [C.c_void_p, C.c_int])
+_native.register("rq_batch_get_tile", C.c_int, [C.c_void_p])
 
 
 class Batch:
@@ -39,7 +41,7 @@ class Batch:
         self._kinds = []
         self._nparams = []
         self._n = []
-        self._keep = []
+        self._layers = []
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -65,6 +67,7 @@ class Batch:
         self._kinds.append("qaoa")
         self._nparams.append(2 * p)
         self._n.append(num_nodes)
+        self._layers.append(p)
         return i.value
 
     def add_vqe(self, n, layers, terms, init_params=None):
@@ -83,6 +86,7 @@ class Batch:
         self._kinds.append("vqe")
         self._nparams.append(n * layers)
         self._n.append(n)
+        self._layers.append(layers)
         return i.value
 
     def evaluate(self):
@@ -121,6 +125,13 @@ class Batch:
     def set_params(self, i, params):
         a = np.ascontiguousarray(params, dtype=np.float64)
         check(lib().rq_batch_set_params(self._h, i, ptr(a)))
+
+    def set_tile(self, tq):
+        """LDS tile (2^tq amplitudes per workgroup) of the states above 13 qubits."""
+        check(lib().rq_batch_set_tile(self._h, int(tq)))
+
+    def tile(self):
+        return int(lib().rq_batch_get_tile(self._h))
 
     def last_ms(self):
         return float(lib().rq_batch_last_ms(self._h))

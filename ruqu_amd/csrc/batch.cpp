@@ -27,7 +27,18 @@ namespace rq {
 
 __global__ void k_batch_eval(const BInst* insts, const BItem* items, int nitems, unsigned* counter,
                              const int32_t* done, const BEdge* edges, const BTerm* terms, const double* params,
-                             const float* ctab, float2* slots, uint64_t slot_amps, double* energy, uint32_t* amax);
+                             double* energy, uint32_t* amax);
+__global__ void k_wide_a(const BInst* insts, const BItem* witems, const int32_t* wmap, const int32_t* wtile0,
+                         const int64_t* wstate, const int32_t* done, const double* params, const float* ctab,
+                         float2* states, int tq, int layer);
+__global__ void k_wide_b(const BInst* insts, const BItem* witems, const int32_t* wmap, const int32_t* wtile0,
+                         const int64_t* wstate, const int32_t* done, const double* params, float2* states, int tq,
+                         int layer);
+__global__ void k_wide_energy(const BInst* insts, const BItem* witems, const int32_t* wmap, const int32_t* wtile0,
+                              const int64_t* wstate, const int32_t* done, const BTerm* terms, const float* ctab,
+                              const float2* states, int tq, BPart* parts);
+__global__ void k_wide_fold(const BInst* insts, const BItem* witems, const int32_t* wtile0, const int32_t* done,
+                            const BPart* parts, int tq, double* energy, uint32_t* amax);
 __global__ void k_cut_table(float* ctab, int n, const BEdge* e, int ne);
 __global__ void k_zero_u32(unsigned* p, int n);
 
@@ -122,12 +133,27 @@ int upload(T** d, const std::vector<T>& h) {
 
 }  // namespace
 
+// Launch plan of one round of circuit evaluations.  Small items (n <= 13)
+// go to the persistent kernel; wide items are sorted by layer count
+// (descending), so the items still active at layer l are a prefix and a
+// layer's launch covers exactly their tiles.
+struct RoundPlan {
+  int small_off = 0, nsmall = 0;  // into all_small
+  int wide_off = 0, nwide = 0;    // into all_wide / all_wtile0 / all_wstate
+  int tile_off = 0, ntiles = 0;   // into all_wmap
+  std::vector<int> layer_grid;    // tiles of the items with more than l layers
+  int64_t state_amps = 0;         // HBM amplitudes of this round's wide states
+};
+
 struct Batch {
   std::vector<BInst> insts;
   std::vector<BEdge> edges;
   std::vector<BTerm> terms;
   std::vector<double> params;
-  bool ready = false;
+  bool ready = false;     // device data uploaded
+  bool planned = false;   // host plan built for the current tile width
+  bool plan_on_device = false;
+  int tq = BATCH_TILE_QUBITS;
   int nwg = 0;
   hipStream_t st = nullptr;
   BInst* d_insts = nullptr;
@@ -135,13 +161,23 @@ struct Batch {
   BTerm* d_terms = nullptr;
   double* d_params = nullptr;
   float* d_ctab = nullptr;
-  float2* d_slots = nullptr;
-  uint64_t slot_amps = 0;
+  int64_t ctab_total = 0;
+  // plan: rounds 0..nrounds-1 of an optimizer iteration, then the base round
   int nrounds = 0;
-  std::vector<int> round_off, round_n;
-  BItem* d_items = nullptr;
-  BItem* d_base_items = nullptr;
-  int nbase = 0;
+  std::vector<RoundPlan> rounds;
+  RoundPlan base;
+  std::vector<BItem> all_small, all_wide;
+  std::vector<int32_t> all_wtile0, all_wmap;
+  std::vector<int64_t> all_wstate;
+  BItem* d_small = nullptr;
+  BItem* d_wide = nullptr;
+  int32_t* d_wtile0 = nullptr;
+  int32_t* d_wmap = nullptr;
+  int64_t* d_wstate = nullptr;
+  float2* d_states = nullptr;
+  int64_t states_cap = 0;  // amplitudes
+  BPart* d_parts = nullptr;
+  int parts_cap = 0;
   int nouts = 0;
   double* d_energy = nullptr;
   uint32_t* d_amax = nullptr;
@@ -151,9 +187,6 @@ struct Batch {
   std::vector<int32_t> out0, outr;
   OptState o{};
   int hist_cap = 0;
-  bool planned = false;
-  int64_t ctab_total = 0;
-  std::vector<BItem> all_items, base_items;
   double* d_best_params = nullptr;
   int32_t* d_zero_done = nullptr;  // all-zero flags for plain evaluation
   hipGraphExec_t exec = nullptr;
@@ -168,14 +201,55 @@ double item_cost(const BInst& in) {
   return in.kind == BK_QAOA ? amps * (in.p + 1) : amps * (in.p + in.nterms);
 }
 
-// Host part of the setup: cut-table offsets, slot size, the per-round work
-// lists and output slots.  No device call.
+// Appends one round's items to the flat plan arrays.
+RoundPlan plan_round(Batch& b, std::vector<BItem> v) {
+  RoundPlan rp;
+  std::vector<BItem> small, wide;
+  for (const BItem& it : v) (b.insts[it.inst].n <= BATCH_LDS_QUBITS ? small : wide).push_back(it);
+  std::stable_sort(small.begin(), small.end(), [&](const BItem& x, const BItem& y) {
+    return item_cost(b.insts[x.inst]) > item_cost(b.insts[y.inst]);
+  });
+  std::stable_sort(wide.begin(), wide.end(), [&](const BItem& x, const BItem& y) {
+    const BInst &a = b.insts[x.inst], &c = b.insts[y.inst];
+    return a.p != c.p ? a.p > c.p : a.n > c.n;
+  });
+  rp.small_off = (int)b.all_small.size();
+  rp.nsmall = (int)small.size();
+  b.all_small.insert(b.all_small.end(), small.begin(), small.end());
+  rp.wide_off = (int)b.all_wide.size();
+  rp.nwide = (int)wide.size();
+  rp.tile_off = (int)b.all_wmap.size();
+  int tiles = 0;
+  int64_t amps = 0;
+  int maxl = 0;
+  for (int j = 0; j < rp.nwide; ++j) maxl = std::max(maxl, b.insts[wide[j].inst].p);
+  rp.layer_grid.assign(maxl, 0);
+  for (int j = 0; j < rp.nwide; ++j) {
+    const BInst& in = b.insts[wide[j].inst];
+    const int nt = 1 << (in.n - b.tq);
+    b.all_wide.push_back(wide[j]);
+    b.all_wtile0.push_back(tiles);
+    b.all_wstate.push_back(amps);
+    for (int t = 0; t < nt; ++t) b.all_wmap.push_back(j);
+    tiles += nt;
+    amps += (int64_t)1 << in.n;
+    for (int l = 0; l < in.p; ++l) rp.layer_grid[l] = tiles;  // sorted by p: a prefix
+  }
+  rp.ntiles = tiles;
+  rp.state_amps = amps;
+  return rp;
+}
+
+// Host part of the setup: cut-table offsets, output slots and the launch
+// plan of every round.  No device call.
 void build_plan(Batch& b) {
   const int ninst = (int)b.insts.size();
+  int nmax = 0;
+  for (const BInst& in : b.insts)
+    if (in.n > BATCH_LDS_QUBITS) nmax = std::max(nmax, in.n);
+  b.tq = std::max(b.tq, (nmax + 1) / 2);  // two tile passes per layer
   b.ctab_total = 0;
-  int maxq_hbm = 0;
   for (BInst& in : b.insts) {
-    if (in.n > BATCH_LDS_QUBITS) maxq_hbm = std::max(maxq_hbm, in.n);
     if (in.kind == BK_QAOA && in.n > BATCH_LDS_QUBITS) {
       in.ctab_off = b.ctab_total;
       b.ctab_total += (int64_t)1 << in.n;
@@ -183,21 +257,20 @@ void build_plan(Batch& b) {
       in.ctab_off = -1;
     }
   }
-  b.slot_amps = maxq_hbm ? (1ull << maxq_hbm) : 0;
   int pmax2 = 0;
   for (const BInst& in : b.insts)
     if (in.kind == BK_QAOA) pmax2 = std::max(pmax2, in.nparams);
   b.nrounds = 1 + pmax2;
   std::vector<std::vector<BItem>> rounds(b.nrounds);
+  std::vector<BItem> base;
   b.out0.assign(ninst, -1);
   b.outr.assign(ninst, -1);
   int out = 0;
-  b.base_items.clear();
   for (int k = 0; k < ninst; ++k) {
     const BInst& in = b.insts[k];
     b.out0[k] = out;
     rounds[0].push_back(BItem{k, 0, -1, out++});
-    b.base_items.push_back(BItem{k, 0, -1, b.out0[k]});
+    base.push_back(BItem{k, 0, -1, b.out0[k]});
     if (in.kind == BK_VQE)
       for (int j = 0; j < in.nparams; ++j) {
         rounds[0].push_back(BItem{k, 1, j, out++});
@@ -214,29 +287,66 @@ void build_plan(Batch& b) {
     }
   }
   b.nouts = out;
-  b.all_items.clear();
-  b.round_off.clear();
-  b.round_n.clear();
-  for (int r = 0; r < b.nrounds; ++r) {
-    auto& v = rounds[r];
-    std::stable_sort(v.begin(), v.end(), [&](const BItem& x, const BItem& y) {
-      return item_cost(b.insts[x.inst]) > item_cost(b.insts[y.inst]);
-    });
-    b.round_off.push_back((int)b.all_items.size());
-    b.round_n.push_back((int)v.size());
-    b.all_items.insert(b.all_items.end(), v.begin(), v.end());
-  }
-  std::stable_sort(b.base_items.begin(), b.base_items.end(), [&](const BItem& x, const BItem& y) {
-    return item_cost(b.insts[x.inst]) > item_cost(b.insts[y.inst]);
-  });
-  b.nbase = (int)b.base_items.size();
+  b.all_small.clear();
+  b.all_wide.clear();
+  b.all_wtile0.clear();
+  b.all_wmap.clear();
+  b.all_wstate.clear();
+  b.rounds.clear();
+  for (int r = 0; r < b.nrounds; ++r) b.rounds.push_back(plan_round(b, rounds[r]));
+  b.base = plan_round(b, base);
   b.planned = true;
+  b.plan_on_device = false;
+}
+
+template <class T>
+void free_dev(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// Uploads the launch plan and sizes the wide-state and partial buffers.
+int upload_plan(Batch& b) {
+  free_dev(b.d_small);
+  free_dev(b.d_wide);
+  free_dev(b.d_wtile0);
+  free_dev(b.d_wmap);
+  free_dev(b.d_wstate);
+  int rc = upload(&b.d_small, b.all_small);
+  if (!rc) rc = upload(&b.d_wide, b.all_wide);
+  if (!rc) rc = upload(&b.d_wtile0, b.all_wtile0);
+  if (!rc) rc = upload(&b.d_wmap, b.all_wmap);
+  if (!rc) rc = upload(&b.d_wstate, b.all_wstate);
+  if (rc) return rc;
+  int64_t amps = b.base.state_amps;
+  int parts = b.base.ntiles;
+  for (const RoundPlan& rp : b.rounds) {
+    amps = std::max(amps, rp.state_amps);
+    parts = std::max(parts, rp.ntiles);
+  }
+  if (amps > b.states_cap) {
+    free_dev(b.d_states);
+    hipError_t e = hipMalloc((void**)&b.d_states, (size_t)amps * sizeof(float2));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      b.states_cap = 0;
+      return fail(RQ_ERR_ALLOC, "batch states: need " + std::to_string((size_t)amps * 8) + " bytes");
+    }
+    b.states_cap = amps;
+  }
+  if (parts > b.parts_cap) {
+    free_dev(b.d_parts);
+    RQ_HIP(hipMalloc((void**)&b.d_parts, (size_t)std::max(parts, 1) * sizeof(BPart)));
+    b.parts_cap = parts;
+  }
+  b.plan_on_device = true;
+  return RQ_OK;
 }
 
 int finalize(Batch& b) {
-  if (b.ready) return RQ_OK;
-  if (b.insts.empty()) return fail(RQ_ERR_ARGUMENT, "empty batch");
   if (!b.planned) build_plan(b);
+  if (b.ready) return b.plan_on_device ? RQ_OK : upload_plan(b);
+  if (b.insts.empty()) return fail(RQ_ERR_ARGUMENT, "empty batch");
   int dev = 0, cus = 0;
   RQ_HIP(hipGetDevice(&dev));
   RQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -244,6 +354,10 @@ int finalize(Batch& b) {
   RQ_HIP(hipStreamCreateWithFlags(&b.st, hipStreamNonBlocking));
   RQ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_eval), hipFuncAttributeMaxDynamicSharedMemorySize,
                              (1 << BATCH_LDS_QUBITS) * 8));
+  RQ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wide_a), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (1 << BATCH_MAX_TILE_QUBITS) * 8));
+  RQ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wide_b), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (1 << BATCH_MAX_TILE_QUBITS) * 8));
   const int ninst = (int)b.insts.size();
   int rc = upload(&b.d_insts, b.insts);
   if (!rc) rc = upload(&b.d_edges, b.edges);
@@ -256,17 +370,9 @@ int finalize(Batch& b) {
     if (in.ctab_off >= 0)
       hipLaunchKernelGGL(k_cut_table, dim3(1024), dim3(BATCH_THREADS), 0, b.st, b.d_ctab + in.ctab_off, in.n,
                          b.d_edges + in.term_off, in.nterms);
-  if (b.slot_amps) {
-    hipError_t e = hipMalloc((void**)&b.d_slots, (size_t)b.nwg * b.slot_amps * sizeof(float2));
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      return fail(RQ_ERR_ALLOC, "batch state slots: need " + std::to_string((size_t)b.nwg * b.slot_amps * 8) + " bytes");
-    }
-  }
-  rc = upload(&b.d_items, b.all_items);
-  if (!rc) rc = upload(&b.d_base_items, b.base_items);
-  if (!rc) rc = upload(&b.d_out0, b.out0);
+  rc = upload(&b.d_out0, b.out0);
   if (!rc) rc = upload(&b.d_outr, b.outr);
+  if (!rc) rc = upload_plan(b);
   if (rc) return rc;
   RQ_HIP(hipMalloc((void**)&b.d_energy, std::max(b.nouts, 1) * sizeof(double)));
   RQ_HIP(hipMalloc((void**)&b.d_amax, std::max(b.nouts, 1) * sizeof(uint32_t)));
@@ -286,11 +392,32 @@ int finalize(Batch& b) {
   return RQ_OK;
 }
 
-void launch_eval(Batch& b, const BItem* items, int n, unsigned* counter, const int32_t* done) {
-  const unsigned grid = (unsigned)std::min(b.nwg, std::max(n, 1));
-  hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BATCH_THREADS), (1 << BATCH_LDS_QUBITS) * 8, b.st, b.d_insts,
-                     items, n, counter, done, b.d_edges, b.d_terms, b.d_params, b.d_ctab, b.d_slots, b.slot_amps,
-                     b.d_energy, b.d_amax);
+// One round: the persistent kernel over the small items, then the layer
+// passes, tile partials and folds of the wide items.
+void enqueue_round(Batch& b, const RoundPlan& rp, unsigned* counter, const int32_t* done) {
+  if (rp.nsmall) {
+    const unsigned grid = (unsigned)std::min(b.nwg, rp.nsmall);
+    hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BATCH_THREADS), (1 << BATCH_LDS_QUBITS) * 8, b.st, b.d_insts,
+                       b.d_small + rp.small_off, rp.nsmall, counter, done, b.d_edges, b.d_terms, b.d_params,
+                       b.d_energy, b.d_amax);
+  }
+  if (!rp.nwide) return;
+  const BItem* wi = b.d_wide + rp.wide_off;
+  const int32_t* wm = b.d_wmap + rp.tile_off;
+  const int32_t* w0 = b.d_wtile0 + rp.wide_off;
+  const int64_t* ws = b.d_wstate + rp.wide_off;
+  const size_t lds = (size_t(1) << b.tq) * sizeof(float2);
+  for (int l = 0; l < (int)rp.layer_grid.size(); ++l) {
+    const dim3 g(rp.layer_grid[l]);
+    hipLaunchKernelGGL(k_wide_a, g, dim3(BATCH_THREADS), lds, b.st, b.d_insts, wi, wm, w0, ws, done, b.d_params,
+                       b.d_ctab, b.d_states, b.tq, l);
+    hipLaunchKernelGGL(k_wide_b, g, dim3(BATCH_THREADS), lds, b.st, b.d_insts, wi, wm, w0, ws, done, b.d_params,
+                       b.d_states, b.tq, l);
+  }
+  hipLaunchKernelGGL(k_wide_energy, dim3(rp.ntiles), dim3(BATCH_THREADS), 0, b.st, b.d_insts, wi, wm, w0, ws, done,
+                     b.d_terms, b.d_ctab, b.d_states, b.tq, b.d_parts);
+  hipLaunchKernelGGL(k_wide_fold, dim3(rp.nwide), dim3(BATCH_THREADS), 0, b.st, b.d_insts, wi, w0, done, b.d_parts,
+                     b.tq, b.d_energy, b.d_amax);
 }
 
 void enqueue_iteration(Batch& b, int max_iter, double qaoa_lr, double vqe_lr, double vqe_thr) {
@@ -298,7 +425,7 @@ void enqueue_iteration(Batch& b, int max_iter, double qaoa_lr, double vqe_lr, do
   const dim3 g1((ninst + 255) / 256), t1(256);
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, b.st, b.d_counters, b.nrounds + 1);
   for (int r = 0; r < b.nrounds; ++r) {
-    launch_eval(b, b.d_items + b.round_off[r], b.round_n[r], b.d_counters + r, b.o.done);
+    enqueue_round(b, b.rounds[r], b.d_counters + r, b.o.done);
     if (r == 0)
       hipLaunchKernelGGL(k_update0, g1, t1, 0, b.st, b.d_insts, ninst, b.d_out0, b.d_energy, b.d_amax, b.d_params,
                          b.o, max_iter, vqe_lr, vqe_thr);
@@ -312,9 +439,10 @@ void enqueue_iteration(Batch& b, int max_iter, double qaoa_lr, double vqe_lr, do
 void free_batch(Batch& b) {
   if (b.st) (void)hipStreamSynchronize(b.st);
   if (b.exec) (void)hipGraphExecDestroy(b.exec);
-  void* ptrs[] = {b.d_insts, b.d_edges, b.d_terms, b.d_params, b.d_ctab, b.d_slots, b.d_items, b.d_base_items,
-                  b.d_energy, b.d_amax, b.d_counters, b.d_out0, b.d_outr, b.o.iter, b.o.hlen, b.o.conv, b.o.done,
-                  b.o.hist, b.o.best, b.o.best_idx, b.d_best_params, b.d_zero_done};
+  void* ptrs[] = {b.d_insts,  b.d_edges,  b.d_terms,  b.d_params, b.d_ctab,     b.d_small,   b.d_wide,
+                  b.d_wtile0, b.d_wmap,   b.d_wstate, b.d_states, b.d_parts,    b.d_energy,  b.d_amax,
+                  b.d_counters, b.d_out0, b.d_outr,   b.o.iter,   b.o.hlen,     b.o.conv,    b.o.done,
+                  b.o.hist,   b.o.best,   b.o.best_idx, b.d_best_params, b.d_zero_done};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (b.st) (void)hipStreamDestroy(b.st);
@@ -367,6 +495,7 @@ int rq_batch_add_qaoa(rq_batch* h, const rq_graph* g, uint32_t p, const double* 
   for (uint32_t l = 0; l < p; ++l) b->params.push_back(init_betas ? init_betas[l] : 0.5);    // qaoa.rs:222
   if (id) *id = (uint32_t)b->insts.size();
   b->insts.push_back(in);
+  b->planned = false;
   return RQ_OK;
 }
 
@@ -402,6 +531,7 @@ int rq_batch_add_vqe(rq_batch* h, uint32_t n, uint32_t layers, const double* coe
   for (int j = 0; j < in.nparams; ++j) b->params.push_back(init_params ? init_params[j] : 0.1);  // vqe.rs:152
   if (id) *id = (uint32_t)b->insts.size();
   b->insts.push_back(in);
+  b->planned = false;
   return RQ_OK;
 }
 
@@ -416,7 +546,7 @@ int rq_batch_evaluate(rq_batch* h, double* energies, uint64_t* argmax) {
   RQ_HIP(hipEventCreate(&e1));
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, b->st, b->d_counters, b->nrounds + 1);
   RQ_HIP(hipEventRecord(e0, b->st));
-  launch_eval(*b, b->d_base_items, b->nbase, b->d_counters + b->nrounds, b->d_zero_done);
+  enqueue_round(*b, b->base, b->d_counters + b->nrounds, b->d_zero_done);
   RQ_HIP(hipEventRecord(e1, b->st));
   RQ_HIP(hipGetLastError());
   RQ_HIP(hipStreamSynchronize(b->st));
@@ -532,31 +662,76 @@ int rq_batch_set_params(rq_batch* h, uint32_t id, const double* params) {
   return RQ_OK;
 }
 
-// Host-only: the records the device kernels receive for the plain evaluation
-// (instances, the sorted base work list), for checks without a GPU.  Pass
-// null pointers to query the counts.
-int rq_batch_plan_export(rq_batch* h, void* insts, void* base_items, int* ninst, int* nbase, uint64_t* slot_amps,
-                         int64_t* ctab_total) {
+// LDS tile width of the wide (n > 13) states: 2^tq amplitudes per
+// workgroup.  Needs ceil(n_max / 2) <= tq <= 13 (two passes per layer).
+int rq_batch_set_tile(rq_batch* h, int tq) {
   Batch* b = (Batch*)h;
-  if (!b || b->insts.empty()) return fail(RQ_ERR_ARGUMENT, "empty batch");
-  if (!b->planned) build_plan(*b);
-  if (ninst) *ninst = (int)b->insts.size();
-  if (nbase) *nbase = b->nbase;
-  if (slot_amps) *slot_amps = b->slot_amps;
-  if (ctab_total) *ctab_total = b->ctab_total;
-  if (insts) std::memcpy(insts, b->insts.data(), b->insts.size() * sizeof(BInst));
-  if (base_items) std::memcpy(base_items, b->base_items.data(), b->base_items.size() * sizeof(BItem));
+  if (!b) return fail(RQ_ERR_ARGUMENT, "null batch");
+  int nmax = 0;
+  for (const BInst& in : b->insts)
+    if (in.n > BATCH_LDS_QUBITS) nmax = std::max(nmax, in.n);
+  if (tq < BATCH_MIN_TILE_QUBITS || tq > BATCH_MAX_TILE_QUBITS || 2 * tq < nmax)
+    return fail(RQ_ERR_ARGUMENT, "tile qubits must be in [max(" + std::to_string(BATCH_MIN_TILE_QUBITS) +
+                                     ", ceil(n_max/2)), " + std::to_string(BATCH_MAX_TILE_QUBITS) + "]");
+  if (tq == b->tq) return RQ_OK;
+  if (b->st) RQ_HIP(hipStreamSynchronize(b->st));
+  b->tq = tq;
+  b->planned = false;
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
   return RQ_OK;
 }
 
-// Host-only: the edge, term and parameter records in device layout.
-int rq_batch_data_export(rq_batch* h, void* edges, void* terms, double* params, size_t* nedges, size_t* nterms,
-                         size_t* nparams) {
+int rq_batch_get_tile(const rq_batch* h) {
+  const Batch* b = (const Batch*)h;
+  return b ? b->tq : -1;
+}
+
+// Host-only (no device call): the launch plan of one round (round < 0: the
+// plain evaluation, else round r of an optimizer iteration), as the device
+// receives it.  info[0..7] = nsmall, nwide, ntiles, nlayers, nrounds, nouts,
+// tq, 0; info64[0..1] = HBM amplitudes of the round's wide states, cut-table
+// floats.  Null arrays are skipped (call once for the counts).
+int rq_batch_plan_export(rq_batch* h, int round, int32_t* info, int64_t* info64, void* small_items,
+                         void* wide_items, int32_t* wtile0, int64_t* wstate, int32_t* wmap, int32_t* layer_grid) {
+  Batch* b = (Batch*)h;
+  if (!b || b->insts.empty()) return fail(RQ_ERR_ARGUMENT, "empty batch");
+  if (!b->planned) build_plan(*b);
+  if (round >= b->nrounds) return fail(RQ_ERR_ARGUMENT, "round out of range");
+  const RoundPlan& rp = round < 0 ? b->base : b->rounds[round];
+  if (info) {
+    const int32_t v[8] = {rp.nsmall, rp.nwide, rp.ntiles, (int32_t)rp.layer_grid.size(), b->nrounds, b->nouts,
+                          b->tq, 0};
+    std::memcpy(info, v, sizeof(v));
+  }
+  if (info64) {
+    info64[0] = rp.state_amps;
+    info64[1] = b->ctab_total;
+  }
+  if (small_items) std::memcpy(small_items, b->all_small.data() + rp.small_off, rp.nsmall * sizeof(BItem));
+  if (wide_items) std::memcpy(wide_items, b->all_wide.data() + rp.wide_off, rp.nwide * sizeof(BItem));
+  if (wtile0) std::memcpy(wtile0, b->all_wtile0.data() + rp.wide_off, rp.nwide * sizeof(int32_t));
+  if (wstate) std::memcpy(wstate, b->all_wstate.data() + rp.wide_off, rp.nwide * sizeof(int64_t));
+  if (wmap) std::memcpy(wmap, b->all_wmap.data() + rp.tile_off, rp.ntiles * sizeof(int32_t));
+  if (layer_grid) std::copy(rp.layer_grid.begin(), rp.layer_grid.end(), layer_grid);
+  return RQ_OK;
+}
+
+// Host-only: instance (48 B, with cut-table offsets), edge (16 B),
+// Pauli-term (32 B) and parameter records in device layout.
+int rq_batch_data_export(rq_batch* h, void* insts, void* edges, void* terms, double* params, size_t* counts) {
   Batch* b = (Batch*)h;
   if (!b) return fail(RQ_ERR_ARGUMENT, "null batch");
-  if (nedges) *nedges = b->edges.size();
-  if (nterms) *nterms = b->terms.size();
-  if (nparams) *nparams = b->params.size();
+  if (!b->insts.empty() && !b->planned) build_plan(*b);
+  if (counts) {
+    counts[0] = b->insts.size();
+    counts[1] = b->edges.size();
+    counts[2] = b->terms.size();
+    counts[3] = b->params.size();
+  }
+  if (insts) std::memcpy(insts, b->insts.data(), b->insts.size() * sizeof(BInst));
   if (edges) std::memcpy(edges, b->edges.data(), b->edges.size() * sizeof(BEdge));
   if (terms) std::memcpy(terms, b->terms.data(), b->terms.size() * sizeof(BTerm));
   if (params) std::memcpy(params, b->params.data(), b->params.size() * sizeof(double));

@@ -1,6 +1,7 @@
-// Batched QAOA / VQE: many small, uneven instances evaluated by one
-// persistent kernel (kernels_batch.hip), optimised by device-side update
-// kernels (batch.cpp), one optimizer iteration per hipGraph replay.
+// Batched QAOA / VQE: many uneven instances evaluated together
+// (kernels_batch.hip: a persistent kernel for LDS-resident states, per-tile
+// launches for larger ones), optimised by device-side update kernels
+// (batch.cpp), one optimizer iteration per hipGraph replay.
 #pragma once
 
 #include <cstdint>
@@ -8,8 +9,10 @@
 namespace rq {
 
 constexpr int BATCH_LDS_QUBITS = 13;       // states up to 2^13 c32 amplitudes live in LDS (64 KiB)
-constexpr int BATCH_TILE_QUBITS = 13;      // tile of the HBM-resident states
-constexpr int BATCH_MAX_QUBITS = 24;
+constexpr int BATCH_TILE_QUBITS = 12;      // default LDS tile of the HBM-resident ("wide") states
+constexpr int BATCH_MIN_TILE_QUBITS = 9;
+constexpr int BATCH_MAX_TILE_QUBITS = 13;
+constexpr int BATCH_MAX_QUBITS = 24;       // two tile passes per layer need n <= 2 * tile qubits
 constexpr int BATCH_THREADS = 256;
 
 enum BatchKind : int32_t { BK_QAOA = 0, BK_VQE = 1 };
@@ -45,6 +48,15 @@ struct BItem {
   int32_t variant;
   int32_t param;
   int32_t out;
+};
+
+// Per-tile partial of a wide item: energy contribution, the tile's largest
+// |a|^2 and its index.
+struct BPart {
+  double e;
+  double pm;
+  uint32_t idx;
+  uint32_t pad;
 };
 
 }  // namespace rq

@@ -1,15 +1,20 @@
-// Persistent batched evaluation of QAOA MaxCut and Ry-CZ VQE circuits.
-//
-// One workgroup evaluates one circuit at a time and takes the next item from
-// a device-wide counter (items are sorted largest first), so 10- to 22-qubit
-// instances of one batch share a single launch without idle tails.
+// Batched evaluation of QAOA MaxCut and Ry-CZ VQE circuits.
 //
 // States are complex-f32; every reduction (energy, Pauli expectations,
 // probabilities for the argmax) accumulates in FP64.
-//   n <= 13: the whole state lives in LDS for the whole circuit.
-//   n >  13: the state lives in this workgroup's HBM slot; each layer streams
-//            it through LDS twice (tiles of the low 13 qubits, then tiles that
-//            hold the high qubits plus the lowest qubits as contiguous runs).
+//   n <= 13 ("small"): k_batch_eval, a persistent kernel; one workgroup
+//            evaluates one circuit at a time with the whole state in LDS and
+//            takes the next work item from a device counter (items sorted
+//            largest first, so uneven sizes share one launch without tails).
+//   n >  13 ("wide"): every state of the round has its own region in HBM and
+//            is spread over many workgroups, one per LDS tile of 2^tq
+//            amplitudes.  Each layer is two launches over all tiles of all
+//            wide items: k_wide_a (tiles of the low tq qubits: cost phase,
+//            rotations on those qubits) and k_wide_b (tiles holding the high
+//            n-tq qubits plus the lowest 2tq-n qubits as contiguous runs:
+//            rotations on the high qubits, the CZ chain).  k_wide_energy
+//            reduces every tile to an FP64 partial and k_wide_fold folds each
+//            item's partials in tile order (deterministic).
 //
 // QAOA (qaoa.rs:125-153 circuit): |+>^n, then per layer the cost phase
 // prod_e Rzz(2 gamma w_e) = exp(i gamma (2 C(i) - W)) per amplitude (C(i) the
@@ -30,7 +35,6 @@ namespace {
 constexpr int B = BATCH_THREADS;
 constexpr int WAVE = 64;
 constexpr int LQ = BATCH_LDS_QUBITS;
-constexpr int TQ = BATCH_TILE_QUBITS;
 constexpr double TWO_PI = 6.283185307179586476925286766559;
 
 __device__ __forceinline__ uint32_t ins0(uint32_t p, uint32_t q) {
@@ -181,86 +185,36 @@ __device__ void qaoa_lds(float2* s, const BInst& in, const BItem& it, const BEdg
   *amax = bi;
 }
 
-__device__ void qaoa_hbm(float2* s, float2* __restrict__ g, const BInst& in, const BItem& it,
-                         const BEdge* __restrict__ edges, const double* __restrict__ params,
-                         const float* __restrict__ ctab_all, Shared& sh, double* energy, uint32_t* amax) {
-  const int n = in.n, p = in.p, h = n - TQ;
-  const uint32_t T = 1u << TQ, N = 1u << n;
-  const float* ctab = ctab_all + in.ctab_off;
-  const float c0 = plus_amp(n);
-  for (int l = 0; l < p; ++l) {
-    const double gamma = param_of(params, in, it, l), beta = param_of(params, in, it, p + l);
-    float c, sn;
-    half_angle(beta, &c, &sn);
-    // pass A: tiles of the low TQ qubits (contiguous), cost phase + Rx on them
-    for (uint32_t t = 0; t < (1u << h); ++t) {
-      const uint32_t base = t << TQ;
-      for (uint32_t i = threadIdx.x; i < T; i += B) {
-        float2 a = l == 0 ? make_float2(c0, 0.0f) : g[base + i];
-        s[i] = phase_mul(a, gamma, (double)ctab[base + i], in.wsum);
-      }
-      __syncthreads();
-      for (int q = 0; q < TQ; ++q) lds_rx(s, TQ, q, c, sn);
-      for (uint32_t i = threadIdx.x; i < T; i += B) g[base + i] = s[i];
-      __syncthreads();
-    }
-    // pass B: tiles holding qubits TQ..n-1 (LDS bits TQ-h..TQ-1) and the low
-    // TQ-h qubits (contiguous runs); the tile id supplies qubits TQ-h..TQ-1.
-    const int lowb = TQ - h;
-    for (uint32_t t = 0; t < (1u << h); ++t) {
-      for (uint32_t k = threadIdx.x; k < T; k += B) {
-        uint32_t lo = k & ((1u << lowb) - 1u), hi = k >> lowb;
-        s[k] = g[lo | (t << lowb) | (hi << TQ)];
-      }
-      __syncthreads();
-      for (int q = 0; q < h; ++q) lds_rx(s, TQ, lowb + q, c, sn);
-      for (uint32_t k = threadIdx.x; k < T; k += B) {
-        uint32_t lo = k & ((1u << lowb) - 1u), hi = k >> lowb;
-        g[lo | (t << lowb) | (hi << TQ)] = s[k];
-      }
-      __syncthreads();
-    }
-  }
-  double acc = 0.0, bv = -1.0;
-  uint32_t bi = 0;
-  for (uint32_t i = threadIdx.x; i < N; i += B) {
-    float2 a = g[i];
-    double pr = (double)a.x * a.x + (double)a.y * a.y;
-    acc += pr * (double)ctab[i];
-    better(bv, bi, (double)(a.x * a.x + a.y * a.y), i);
-  }
-  *energy = block_sum_all(acc, sh.red);
-  block_argmax(bv, bi, sh.av, sh.ai);
-  *amax = bi;
-}
-
 // ----------------------------------------------------------------- VQE ----
 __device__ __forceinline__ float2 cz_chain(float2 a, uint32_t i, int n) {
   const uint32_t m = (1u << (n - 1)) - 1u;
   return (__popc(i & (i >> 1) & m) & 1) ? make_float2(-a.x, -a.y) : a;
 }
 
-// sum_k c_k <P_k> over a state of 2^n amplitudes at `a` (LDS or HBM).
-__device__ double vqe_energy(const float2* a, int n, const BTerm* __restrict__ terms, int nt, Shared& sh) {
+// Re((conj(a_j) * phase) * a_i) for Pauli term t at index i (state.rs:362-400
+// semantics: X/Y flip bit q, Z/Y sign from bit q of i, i^{nY} overall).
+__device__ __forceinline__ double pauli_contrib(float2 ai, float2 aj, uint32_t i, const BTerm& t) {
+  double s = (__popc(i & (uint32_t)t.zy) & 1) ? -1.0 : 1.0;
+  double pr, pim;
+  switch (t.ny & 3) {
+    case 0: pr = s; pim = 0.0; break;
+    case 1: pr = 0.0; pim = s; break;
+    case 2: pr = -s; pim = 0.0; break;
+    default: pr = 0.0; pim = -s; break;
+  }
+  double cr = aj.x, ci = -(double)aj.y;
+  double tr = cr * pr - ci * pim, ti = cr * pim + ci * pr;
+  return tr * (double)ai.x - ti * (double)ai.y;
+}
+
+// sum_k c_k <P_k> over an LDS-resident state of 2^n amplitudes.
+__device__ double vqe_energy_lds(const float2* a, int n, const BTerm* __restrict__ terms, int nt, Shared& sh) {
   const uint32_t N = 1u << n;
   double energy = 0.0;
   for (int k = 0; k < nt; ++k) {
     const BTerm t = terms[k];
     double acc = 0.0;
-    for (uint32_t i = threadIdx.x; i < N; i += B) {
-      float2 ai = a[i], aj = a[i ^ (uint32_t)t.x];
-      double s = (__popc(i & (uint32_t)t.zy) & 1) ? -1.0 : 1.0;
-      double pr, pim;
-      switch (t.ny & 3) {
-        case 0: pr = s; pim = 0.0; break;
-        case 1: pr = 0.0; pim = s; break;
-        case 2: pr = -s; pim = 0.0; break;
-        default: pr = 0.0; pim = -s; break;
-      }
-      double cr = aj.x, ci = -(double)aj.y;
-      double tr = cr * pr - ci * pim, ti = cr * pim + ci * pr;
-      acc += tr * (double)ai.x - ti * (double)ai.y;
-    }
+    for (uint32_t i = threadIdx.x; i < N; i += B) acc += pauli_contrib(a[i], a[i ^ (uint32_t)t.x], i, t);
     energy += t.c * block_sum_all(acc, sh.red);
   }
   return energy;
@@ -281,48 +235,7 @@ __device__ void vqe_lds(float2* s, const BInst& in, const BItem& it, const BTerm
     for (uint32_t i = threadIdx.x; i < N; i += B) s[i] = cz_chain(s[i], i, n);
     __syncthreads();
   }
-  *energy = vqe_energy(s, n, terms + in.term_off, in.nterms, sh);
-}
-
-__device__ void vqe_hbm(float2* s, float2* __restrict__ g, const BInst& in, const BItem& it,
-                        const BTerm* __restrict__ terms, const double* __restrict__ params, Shared& sh,
-                        double* energy) {
-  const int n = in.n, L = in.p, h = n - TQ, lowb = TQ - h;
-  const uint32_t T = 1u << TQ;
-  for (int l = 0; l < L; ++l) {
-    for (uint32_t t = 0; t < (1u << h); ++t) {  // pass A: Ry on qubits 0..TQ-1
-      const uint32_t base = t << TQ;
-      for (uint32_t i = threadIdx.x; i < T; i += B)
-        s[i] = l == 0 ? make_float2(base + i == 0 ? 1.0f : 0.0f, 0.0f) : g[base + i];
-      __syncthreads();
-      for (int q = 0; q < TQ; ++q) {
-        float c, sn;
-        half_angle(param_of(params, in, it, l * n + q) / 2.0, &c, &sn);
-        lds_ry(s, TQ, q, c, sn);
-      }
-      for (uint32_t i = threadIdx.x; i < T; i += B) g[base + i] = s[i];
-      __syncthreads();
-    }
-    for (uint32_t t = 0; t < (1u << h); ++t) {  // pass B: Ry on qubits TQ..n-1, then the CZ chain
-      for (uint32_t k = threadIdx.x; k < T; k += B) {
-        uint32_t lo = k & ((1u << lowb) - 1u), hi = k >> lowb;
-        s[k] = g[lo | (t << lowb) | (hi << TQ)];
-      }
-      __syncthreads();
-      for (int q = 0; q < h; ++q) {
-        float c, sn;
-        half_angle(param_of(params, in, it, l * n + TQ + q) / 2.0, &c, &sn);
-        lds_ry(s, TQ, lowb + q, c, sn);
-      }
-      for (uint32_t k = threadIdx.x; k < T; k += B) {
-        uint32_t lo = k & ((1u << lowb) - 1u), hi = k >> lowb;
-        uint32_t gi = lo | (t << lowb) | (hi << TQ);
-        g[gi] = cz_chain(s[k], gi, n);
-      }
-      __syncthreads();
-    }
-  }
-  *energy = vqe_energy(g, n, terms + in.term_off, in.nterms, sh);
+  *energy = vqe_energy_lds(s, n, terms + in.term_off, in.nterms, sh);
 }
 
 }  // namespace
@@ -331,12 +244,9 @@ __global__ __launch_bounds__(B) void k_batch_eval(const BInst* __restrict__ inst
                                                   int nitems, unsigned* __restrict__ counter,
                                                   const int32_t* __restrict__ done, const BEdge* __restrict__ edges,
                                                   const BTerm* __restrict__ terms, const double* __restrict__ params,
-                                                  const float* __restrict__ ctab, float2* __restrict__ slots,
-                                                  uint64_t slot_amps, double* __restrict__ energy,
-                                                  uint32_t* __restrict__ amax) {
+                                                  double* __restrict__ energy, uint32_t* __restrict__ amax) {
   extern __shared__ float2 lds_state[];
   __shared__ Shared sh;
-  float2* g = slots + (uint64_t)blockIdx.x * slot_amps;
   // Every branch of this loop is uniform across a wave (wave index and item
   // index come through readfirstlane).  A branch on the lane index here
   // (`if (threadIdx.x == 0)` around the ticket or the result store) lets the
@@ -369,17 +279,164 @@ __global__ __launch_bounds__(B) void k_batch_eval(const BInst* __restrict__ inst
     }
     double en = 0.0;
     uint32_t am = 0;
-    if (in.kind == BK_QAOA) {
-      if (in.n <= LQ) qaoa_lds(lds_state, in, it, edges, params, sh, &en, &am);
-      else qaoa_hbm(lds_state, g, in, it, edges, params, ctab, sh, &en, &am);
-    } else {
-      if (in.n <= LQ) vqe_lds(lds_state, in, it, terms, params, sh, &en);
-      else vqe_hbm(lds_state, g, in, it, terms, params, sh, &en);
-    }
+    if (in.kind == BK_QAOA) qaoa_lds(lds_state, in, it, edges, params, sh, &en, &am);
+    else vqe_lds(lds_state, in, it, terms, params, sh, &en);
     if (wid == 0) {  // en and am are block-uniform: wave 0 stores them
       energy[it.out] = en;
       amax[it.out] = am;
     }
+  }
+}
+
+// ------------------------------------------------------------ wide items ----
+// Block b handles tile (b - wtile0[j]) of wide item j = wmap[b]; its state
+// starts at states + wstate[j].  All indices come from blockIdx (uniform).
+
+__global__ __launch_bounds__(B) void k_wide_a(const BInst* __restrict__ insts, const BItem* __restrict__ witems,
+                                              const int32_t* __restrict__ wmap, const int32_t* __restrict__ wtile0,
+                                              const int64_t* __restrict__ wstate, const int32_t* __restrict__ done,
+                                              const double* __restrict__ params, const float* __restrict__ ctab,
+                                              float2* __restrict__ states, int tq, int layer) {
+  extern __shared__ float2 s[];
+  const int j = wmap[blockIdx.x];
+  const BItem it = witems[j];
+  const BInst in = insts[it.inst];
+  if (done[it.inst]) return;
+  const uint32_t T = 1u << tq, base = (uint32_t)(blockIdx.x - wtile0[j]) << tq;
+  float2* __restrict__ g = states + wstate[j];
+  if (in.kind == BK_QAOA) {
+    const double gamma = param_of(params, in, it, layer), beta = param_of(params, in, it, in.p + layer);
+    const float* __restrict__ ct = ctab + in.ctab_off;
+    const float c0 = plus_amp(in.n);
+    float c, sn;
+    half_angle(beta, &c, &sn);
+    for (uint32_t i = threadIdx.x; i < T; i += B) {
+      const float2 a = layer == 0 ? make_float2(c0, 0.0f) : g[base + i];
+      s[i] = phase_mul(a, gamma, (double)ct[base + i], in.wsum);
+    }
+    __syncthreads();
+    for (int q = 0; q < tq; ++q) lds_rx(s, tq, q, c, sn);
+  } else {
+    for (uint32_t i = threadIdx.x; i < T; i += B)
+      s[i] = layer == 0 ? make_float2(base + i == 0 ? 1.0f : 0.0f, 0.0f) : g[base + i];
+    __syncthreads();
+    for (int q = 0; q < tq; ++q) {
+      float c, sn;
+      half_angle(param_of(params, in, it, layer * in.n + q) / 2.0, &c, &sn);
+      lds_ry(s, tq, q, c, sn);
+    }
+  }
+  for (uint32_t i = threadIdx.x; i < T; i += B) g[base + i] = s[i];
+}
+
+__global__ __launch_bounds__(B) void k_wide_b(const BInst* __restrict__ insts, const BItem* __restrict__ witems,
+                                              const int32_t* __restrict__ wmap, const int32_t* __restrict__ wtile0,
+                                              const int64_t* __restrict__ wstate, const int32_t* __restrict__ done,
+                                              const double* __restrict__ params, float2* __restrict__ states, int tq,
+                                              int layer) {
+  extern __shared__ float2 s[];
+  const int j = wmap[blockIdx.x];
+  const BItem it = witems[j];
+  const BInst in = insts[it.inst];
+  if (done[it.inst]) return;
+  const int n = in.n, h = n - tq, lowb = tq - h;
+  const uint32_t T = 1u << tq, t = (uint32_t)(blockIdx.x - wtile0[j]), lom = (1u << lowb) - 1u;
+  float2* __restrict__ g = states + wstate[j];
+  for (uint32_t k = threadIdx.x; k < T; k += B) s[k] = g[(k & lom) | (t << lowb) | ((k >> lowb) << tq)];
+  __syncthreads();
+  if (in.kind == BK_QAOA) {
+    float c, sn;
+    half_angle(param_of(params, in, it, in.p + layer), &c, &sn);
+    for (int q = 0; q < h; ++q) lds_rx(s, tq, lowb + q, c, sn);
+    for (uint32_t k = threadIdx.x; k < T; k += B) g[(k & lom) | (t << lowb) | ((k >> lowb) << tq)] = s[k];
+  } else {
+    for (int q = 0; q < h; ++q) {
+      float c, sn;
+      half_angle(param_of(params, in, it, layer * n + tq + q) / 2.0, &c, &sn);
+      lds_ry(s, tq, lowb + q, c, sn);
+    }
+    for (uint32_t k = threadIdx.x; k < T; k += B) {
+      const uint32_t gi = (k & lom) | (t << lowb) | ((k >> lowb) << tq);
+      g[gi] = cz_chain(s[k], gi, n);
+    }
+  }
+}
+
+// Per-tile partial: QAOA sum |a|^2 C(i) and the tile's argmax of |a|^2; VQE
+// sum_k c_k Re(conj(a_{i^x_k}) phase_k(i) a_i) over the tile's indices i.
+__global__ __launch_bounds__(B) void k_wide_energy(const BInst* __restrict__ insts, const BItem* __restrict__ witems,
+                                                   const int32_t* __restrict__ wmap,
+                                                   const int32_t* __restrict__ wtile0,
+                                                   const int64_t* __restrict__ wstate,
+                                                   const int32_t* __restrict__ done, const BTerm* __restrict__ terms,
+                                                   const float* __restrict__ ctab, const float2* __restrict__ states,
+                                                   int tq, BPart* __restrict__ parts) {
+  __shared__ Shared sh;
+  const int j = wmap[blockIdx.x];
+  const BItem it = witems[j];
+  const BInst in = insts[it.inst];
+  if (done[it.inst]) return;
+  const uint32_t T = 1u << tq, base = (uint32_t)(blockIdx.x - wtile0[j]) << tq;
+  const float2* __restrict__ g = states + wstate[j];
+  const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+  double acc = 0.0, bv = -1.0;
+  uint32_t bi = 0;
+  if (in.kind == BK_QAOA) {
+    const float* __restrict__ ct = ctab + in.ctab_off;
+    for (uint32_t i = threadIdx.x; i < T; i += B) {
+      const float2 a = g[base + i];
+      acc += ((double)a.x * a.x + (double)a.y * a.y) * (double)ct[base + i];
+      better(bv, bi, (double)(a.x * a.x + a.y * a.y), base + i);
+    }
+    block_argmax(bv, bi, sh.av, sh.ai);
+  } else {
+    const BTerm* __restrict__ tt = terms + in.term_off;
+    for (int k = 0; k < in.nterms; ++k) {
+      const BTerm t = tt[k];
+      double a_k = 0.0;
+      for (uint32_t i = threadIdx.x; i < T; i += B) {
+        const uint32_t gi = base + i;
+        a_k += pauli_contrib(g[gi], g[gi ^ (uint32_t)t.x], gi, t);
+      }
+      acc += t.c * a_k;
+    }
+  }
+  const double e = block_sum_all(acc, sh.red);
+  if (wid == 0) parts[blockIdx.x] = BPart{e, bv, bi, 0u};
+}
+
+// One block per wide item: folds its tiles' partials (fixed assignment of
+// tiles to threads and a fixed reduction tree, so the result does not depend
+// on scheduling) into energy[out] and amax[out].
+__global__ __launch_bounds__(B) void k_wide_fold(const BInst* __restrict__ insts, const BItem* __restrict__ witems,
+                                                 const int32_t* __restrict__ wtile0, const int32_t* __restrict__ done,
+                                                 const BPart* __restrict__ parts, int tq, double* __restrict__ energy,
+                                                 uint32_t* __restrict__ amax) {
+  __shared__ Shared sh;
+  const int j = blockIdx.x;
+  const BItem it = witems[j];
+  const BInst in = insts[it.inst];
+  const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+  if (done[it.inst]) {
+    if (wid == 0) {
+      energy[it.out] = 0.0;
+      amax[it.out] = 0;
+    }
+    return;
+  }
+  const int ntiles = 1 << (in.n - tq), t0 = wtile0[j];
+  double acc = 0.0, bv = -1.0;
+  uint32_t bi = 0;
+  for (int t = threadIdx.x; t < ntiles; t += B) {
+    const BPart q = parts[t0 + t];
+    acc += q.e;
+    better(bv, bi, q.pm, q.idx);
+  }
+  const double e = block_sum_all(acc, sh.red);
+  block_argmax(bv, bi, sh.av, sh.ai);
+  if (wid == 0) {
+    energy[it.out] = e;
+    amax[it.out] = in.kind == BK_QAOA ? bi : 0u;
   }
 }
 

@@ -82,8 +82,7 @@ class _Evaluator:
         if self.state is None:
             self.state = (QuantumState.new_with_seed(cfg.num_qubits, cfg.seed, self.precision)
                           if cfg.seed is not None else QuantumState.new(cfg.num_qubits, self.precision))
-        else:
-            self.state.set_zero()
+        # rq_run_circuit starts from |0...0> on the reused state
         Simulator.run_with_config(circ, SimConfig(seed=cfg.seed, precision=self.precision), state=self.state)
         return self.state.expectation_hamiltonian(cfg.hamiltonian)
 

@@ -1,5 +1,8 @@
-// CPU run of the batched kernel source (tests only): the kernel file is
-// included after the shim; launches run block by block with 256 threads.
+// CPU run of the batched kernels' source (tests only): the kernel file is
+// included after the shim; each launch runs its blocks one after another,
+// 256 host threads per block.
+#include <functional>
+
 #include "hip_cpu_shim.h"
 
 thread_local Dim3 threadIdx, blockIdx;
@@ -14,16 +17,13 @@ unsigned char* lds = nullptr;
 
 #include "batch_kernel_src.inc"
 
-extern "C" {
+namespace {
 
-// Runs k_batch_eval with `grid` blocks (sequentially, 256 threads each).
-int emu_batch_eval(const rq::BInst* insts, const rq::BItem* items, int nitems, const int32_t* done,
-                   const rq::BEdge* edges, const rq::BTerm* terms, const double* params, const float* ctab,
-                   float2* slots, uint64_t slot_amps, double* energy, uint32_t* amax, int grid) {
+std::vector<unsigned char> g_lds((1 << rq::BATCH_MAX_TILE_QUBITS) * 8);
+
+void run_grid(int grid, const std::function<void()>& body) {
   const int B = rq::BATCH_THREADS;
-  unsigned counter = 0;
-  std::vector<unsigned char> lds((1 << rq::BATCH_LDS_QUBITS) * 8);
-  emu::lds = lds.data();
+  emu::lds = g_lds.data();
   emu::xchg.assign(B, 0.0);
   blockDim.x = B;
   gridDim.x = grid;
@@ -40,11 +40,47 @@ int emu_batch_eval(const rq::BInst* insts, const rq::BItem* items, int nitems, c
       th.emplace_back([&, t, b] {
         threadIdx.x = t;
         blockIdx.x = b;
-        rq::k_batch_eval(insts, items, nitems, &counter, done, edges, terms, params, ctab, slots, slot_amps, energy,
-                         amax);
+        body();
       });
     for (auto& x : th) x.join();
   }
+}
+
+}  // namespace
+
+extern "C" {
+
+int emu_batch_eval(const rq::BInst* insts, const rq::BItem* items, int nitems, const int32_t* done,
+                   const rq::BEdge* edges, const rq::BTerm* terms, const double* params, double* energy,
+                   uint32_t* amax, int grid) {
+  unsigned counter = 0;
+  run_grid(grid, [&] { rq::k_batch_eval(insts, items, nitems, &counter, done, edges, terms, params, energy, amax); });
+  return 0;
+}
+
+int emu_wide_a(int grid, const rq::BInst* insts, const rq::BItem* wi, const int32_t* wmap, const int32_t* wtile0,
+               const int64_t* wstate, const int32_t* done, const double* params, const float* ctab, float2* states,
+               int tq, int layer) {
+  run_grid(grid, [&] { rq::k_wide_a(insts, wi, wmap, wtile0, wstate, done, params, ctab, states, tq, layer); });
+  return 0;
+}
+
+int emu_wide_b(int grid, const rq::BInst* insts, const rq::BItem* wi, const int32_t* wmap, const int32_t* wtile0,
+               const int64_t* wstate, const int32_t* done, const double* params, float2* states, int tq, int layer) {
+  run_grid(grid, [&] { rq::k_wide_b(insts, wi, wmap, wtile0, wstate, done, params, states, tq, layer); });
+  return 0;
+}
+
+int emu_wide_energy(int grid, const rq::BInst* insts, const rq::BItem* wi, const int32_t* wmap,
+                    const int32_t* wtile0, const int64_t* wstate, const int32_t* done, const rq::BTerm* terms,
+                    const float* ctab, const float2* states, int tq, rq::BPart* parts) {
+  run_grid(grid, [&] { rq::k_wide_energy(insts, wi, wmap, wtile0, wstate, done, terms, ctab, states, tq, parts); });
+  return 0;
+}
+
+int emu_wide_fold(int grid, const rq::BInst* insts, const rq::BItem* wi, const int32_t* wtile0, const int32_t* done,
+                  const rq::BPart* parts, int tq, double* energy, uint32_t* amax) {
+  run_grid(grid, [&] { rq::k_wide_fold(insts, wi, wtile0, done, parts, tq, energy, amax); });
   return 0;
 }
 

@@ -1,8 +1,7 @@
 """Runs the batched kernel's source on the CPU (tests/emu/libbatch_emu.so)
-over the records that ruqu_amd/csrc/batch.cpp hands the device for
-rq_batch_evaluate: the instance, edge, term and parameter arrays and the
-sorted work list, exported by rq_batch_plan_export / rq_batch_data_export
-(host-only calls, no GPU).  Test infrastructure only."""
+over the records and launch plan that ruqu_amd/csrc/batch.cpp hands the
+device for rq_batch_evaluate, exported by rq_batch_plan_export /
+rq_batch_data_export (host-only calls, no GPU).  Test infrastructure only."""
 import ctypes as C
 import os
 
@@ -24,60 +23,92 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+PART = np.dtype([("e", "<f8"), ("pm", "<f8"), ("idx", "<u4"), ("pad", "<u4")])
+I32P, I64P = C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+
+
 def _emu():
     from tests.emu import build_emu
     srcs = [build_emu.SRC, os.path.join(HERE, "batch_emu.cpp"), os.path.join(HERE, "hip_cpu_shim.h")]
     if not os.path.exists(LIB) or os.path.getmtime(LIB) < max(os.path.getmtime(f) for f in srcs):
         build_emu.build()
     lib = C.CDLL(LIB)
-    lib.emu_cut_table.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
-    lib.emu_batch_eval.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                   C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_int]
+    vp, i = C.c_void_p, C.c_int
+    lib.emu_cut_table.argtypes = [vp, i, vp, i]
+    lib.emu_batch_eval.argtypes = [vp, vp, i] + [vp] * 6 + [i]
+    lib.emu_wide_a.argtypes = [i] + [vp] * 9 + [i, i]
+    lib.emu_wide_b.argtypes = [i] + [vp] * 8 + [i, i]
+    lib.emu_wide_energy.argtypes = [i] + [vp] * 9 + [i, vp]
+    lib.emu_wide_fold.argtypes = [i] + [vp] * 5 + [i, vp, vp]
     return lib
 
 
-def export(batch):
-    """The device records of a ruqu_amd.batch.Batch, as numpy arrays."""
+def data(batch):
+    """Instance, edge, term and parameter records of a ruqu_amd.batch.Batch."""
     lib = _native.lib()
-    lib.rq_batch_plan_export.argtypes = [C.c_void_p] * 3 + [C.POINTER(C.c_int)] * 2 + [C.POINTER(C.c_uint64),
-                                                                                      C.POINTER(C.c_int64)]
-    lib.rq_batch_data_export.argtypes = [C.c_void_p] * 4 + [C.POINTER(C.c_size_t)] * 3
-    ni, nb, sa, ct = C.c_int(), C.c_int(), C.c_uint64(), C.c_int64()
-    assert lib.rq_batch_plan_export(batch._h, None, None, C.byref(ni), C.byref(nb), C.byref(sa), C.byref(ct)) == 0
-    insts, items = np.zeros(ni.value, dtype=INST), np.zeros(nb.value, dtype=ITEM)
-    assert lib.rq_batch_plan_export(batch._h, _p(insts), _p(items), None, None, None, None) == 0
-    ne, nt, npm = C.c_size_t(), C.c_size_t(), C.c_size_t()
-    assert lib.rq_batch_data_export(batch._h, None, None, None, C.byref(ne), C.byref(nt), C.byref(npm)) == 0
-    edges, terms, params = np.zeros(ne.value, dtype=EDGE), np.zeros(nt.value, dtype=TERM), np.zeros(npm.value)
-    assert lib.rq_batch_data_export(batch._h, _p(edges), _p(terms), _p(params), None, None, None) == 0
-    return dict(insts=insts, items=items, edges=edges, terms=terms, params=params, slot_amps=sa.value,
-                ctab_total=ct.value)
+    lib.rq_batch_data_export.argtypes = [C.c_void_p] * 5 + [C.POINTER(C.c_size_t)]
+    cnt = (C.c_size_t * 4)()
+    assert lib.rq_batch_data_export(batch._h, None, None, None, None, cnt) == 0
+    insts, edges = np.zeros(cnt[0], dtype=INST), np.zeros(max(cnt[1], 1), dtype=EDGE)
+    terms, params = np.zeros(max(cnt[2], 1), dtype=TERM), np.zeros(max(cnt[3], 1))
+    assert lib.rq_batch_data_export(batch._h, _p(insts), _p(edges), _p(terms), _p(params), cnt) == 0
+    return dict(insts=insts, edges=edges, terms=terms, params=params)
 
 
-def evaluate(batch, grid=4):
+def plan(batch, rnd=-1):
+    """The launch plan of one round (-1: plain evaluation) as batch.cpp builds it."""
+    lib = _native.lib()
+    lib.rq_batch_plan_export.argtypes = [C.c_void_p, C.c_int, I32P, I64P, C.c_void_p, C.c_void_p, I32P, I64P, I32P,
+                                         I32P]
+    info = np.zeros(8, dtype=np.int32)
+    info64 = np.zeros(2, dtype=np.int64)
+    ip = lambda a: a.ctypes.data_as(I32P)  # noqa: E731
+    lp = lambda a: a.ctypes.data_as(I64P)  # noqa: E731
+    assert lib.rq_batch_plan_export(batch._h, rnd, ip(info), lp(info64), None, None, None, None, None, None) == 0
+    nsmall, nwide, ntiles, nlayers = (int(x) for x in info[:4])
+    r = dict(small=np.zeros(max(nsmall, 1), dtype=ITEM)[:nsmall], wide=np.zeros(max(nwide, 1), dtype=ITEM)[:nwide],
+             wtile0=np.zeros(max(nwide, 1), dtype=np.int32), wstate=np.zeros(max(nwide, 1), dtype=np.int64),
+             wmap=np.zeros(max(ntiles, 1), dtype=np.int32), layer_grid=np.zeros(max(nlayers, 1), dtype=np.int32))
+    assert lib.rq_batch_plan_export(batch._h, rnd, None, None, _p(r["small"]), _p(r["wide"]), ip(r["wtile0"]),
+                                    lp(r["wstate"]), ip(r["wmap"]), ip(r["layer_grid"])) == 0
+    r.update(nsmall=nsmall, nwide=nwide, ntiles=ntiles, nlayers=nlayers, nrounds=int(info[4]), nouts=int(info[5]),
+             tq=int(info[6]), state_amps=int(info64[0]), ctab_total=int(info64[1]))
+    r["layer_grid"] = r["layer_grid"][:nlayers]
+    return r
+
+
+def evaluate(batch, grid=3):
     """Energies and argmax indices per instance, as rq_batch_evaluate returns
-    them, with the kernel run block after block on host threads."""
-    r = export(batch)
+    them: the launches of batch.cpp's enqueue_round run block after block on
+    host threads over the exported plan."""
+    d, r = data(batch), plan(batch, -1)
     lib = _emu()
-    insts, items = r["insts"], r["items"]
-    ea = r["edges"] if len(r["edges"]) else np.zeros(1, dtype=EDGE)
-    ta = r["terms"] if len(r["terms"]) else np.zeros(1, dtype=TERM)
+    insts, ea, ta, pa = d["insts"], d["edges"], d["terms"], d["params"]
     ctab = np.zeros(max(r["ctab_total"], 1), dtype=np.float32)
     for k in range(len(insts)):
         off = int(insts[k]["ctab_off"])
         if off >= 0:
             lib.emu_cut_table(_p(ctab[off:]), int(insts[k]["n"]), _p(ea[int(insts[k]["term_off"]):]),
                               int(insts[k]["nterms"]))
-    nouts = int(items["out"].max()) + 1
-    slots = np.zeros(max(grid * r["slot_amps"], 1), dtype=np.complex64)
-    energy = np.zeros(nouts, dtype=np.float64)
-    amax = np.zeros(nouts, dtype=np.uint32)
+    energy = np.zeros(r["nouts"], dtype=np.float64)
+    amax = np.zeros(r["nouts"], dtype=np.uint32)
     done = np.zeros(len(insts), dtype=np.int32)
-    lib.emu_batch_eval(_p(insts), _p(items), len(items), _p(done), _p(ea), _p(ta), _p(r["params"]), _p(ctab),
-                       _p(slots), r["slot_amps"], _p(energy), _p(amax), grid)
+    if r["nsmall"]:
+        lib.emu_batch_eval(_p(insts), _p(r["small"]), r["nsmall"], _p(done), _p(ea), _p(ta), _p(pa), _p(energy),
+                           _p(amax), min(grid, r["nsmall"]))
+    if r["nwide"]:
+        states = np.zeros(max(r["state_amps"], 1), dtype=np.complex64)
+        parts = np.zeros(r["ntiles"], dtype=PART)
+        w = (_p(insts), _p(r["wide"]), _p(r["wmap"]), _p(r["wtile0"]), _p(r["wstate"]), _p(done))
+        for layer, g in enumerate(r["layer_grid"]):
+            lib.emu_wide_a(int(g), *w, _p(pa), _p(ctab), _p(states), r["tq"], layer)
+            lib.emu_wide_b(int(g), *w, _p(pa), _p(states), r["tq"], layer)
+        lib.emu_wide_energy(r["ntiles"], *w, _p(ta), _p(ctab), _p(states), r["tq"], _p(parts))
+        lib.emu_wide_fold(r["nwide"], _p(insts), _p(r["wide"]), _p(r["wtile0"]), _p(done), _p(parts), r["tq"],
+                          _p(energy), _p(amax))
     en = np.zeros(len(insts))
     am = np.zeros(len(insts), dtype=np.uint64)
-    for it in items:
+    for it in list(r["small"]) + list(r["wide"]):
         en[it["inst"]] = energy[it["out"]]
         am[it["inst"]] = amax[it["out"]]
     return en, am

@@ -1,6 +1,7 @@
 """Builds tests/emu/libbatch_emu.so: the batched kernel's source compiled for
 the host with the thread-emulation shim (test infrastructure, CPU only)."""
 import os
+import re
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -13,7 +14,7 @@ def build():
     s = open(SRC).read()
     s = s.replace("#include <hip/hip_runtime.h>", "")
     s = s.replace('#include "batch.h"', '#include "%s"' % os.path.join(ROOT, "ruqu_amd", "csrc", "batch.h"))
-    s = s.replace("extern __shared__ float2 lds_state[];", "float2* lds_state = (float2*)emu::lds;")
+    s = re.sub(r"extern __shared__ float2 (\w+)\[\];", r"float2* \1 = (float2*)emu::lds;", s)
     s = s.replace("__shared__ Shared sh;", "static Shared sh;")
     s += "\nnamespace rq { inline double emu_cut_of(uint64_t i, const BEdge* e, int ne) { return cut_of(i, e, ne); } }\n"
     inc = os.path.join(HERE, "batch_kernel_src.inc")

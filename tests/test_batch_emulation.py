@@ -8,6 +8,7 @@ with FP64 reductions, |dE| <= 1e-5 * (1 + sum |weights or coefficients|)."""
 import math
 
 import numpy as np
+import pytest
 
 from ruqu_amd import workloads
 from ruqu_amd.batch import Batch
@@ -48,12 +49,43 @@ def test_batch_plan_order_and_layout():
     e2, w2 = workloads.erdos_renyi_graph(6, 0.5, 6)
     b.add_qaoa(6, e2, w2, 1)
     b.add_vqe(12, 3, workloads.random_pauli_hamiltonian(12, 5, 3, 7))
-    r = batch_emu.export(b)
-    ins, items = r["insts"], r["items"]
-    assert list(ins["n"]) == [16, 6, 12]
-    assert list(ins["nparams"]) == [4, 2, 36]
-    assert ins[0]["ctab_off"] == 0 and ins[1]["ctab_off"] == -1 and ins[2]["ctab_off"] == -1
-    assert r["ctab_total"] == 1 << 16 and r["slot_amps"] == 1 << 16
-    assert list(items["inst"]) == [0, 2, 1]  # largest state first
+    e3, w3 = workloads.erdos_renyi_graph(14, 0.5, 8)
+    b.add_qaoa(14, e3, w3, 3)
+    d = batch_emu.data(b)
+    ins = d["insts"]
+    assert list(ins["n"]) == [16, 6, 12, 14]
+    assert list(ins["nparams"]) == [4, 2, 36, 6]
+    assert list(ins["ctab_off"]) == [0, -1, -1, 1 << 16]
     assert np.isclose(ins[0]["wsum"], sum(w))
-    assert len(r["params"]) == 4 + 2 + 36 and np.all(r["params"][:6] == 0.5) and np.all(r["params"][6:] == 0.1)
+    assert len(d["params"]) == 4 + 2 + 36 + 6
+    assert np.all(d["params"][:6] == 0.5) and np.all(d["params"][6:42] == 0.1) and np.all(d["params"][42:] == 0.5)
+    r = batch_emu.plan(b, -1)
+    tq = r["tq"]
+    assert tq == 12 and r["ctab_total"] == (1 << 16) + (1 << 14)
+    assert list(r["small"]["inst"]) == [2, 1]  # largest state first
+    assert list(r["wide"]["inst"]) == [3, 0]  # most layers first: layer l launches a prefix
+    assert list(r["wtile0"]) == [0, 1 << (14 - tq)]
+    assert list(r["wstate"]) == [0, 1 << 14]
+    assert r["ntiles"] == (1 << (14 - tq)) + (1 << (16 - tq)) and len(r["wmap"]) == r["ntiles"]
+    assert list(r["layer_grid"]) == [r["ntiles"], r["ntiles"], 1 << (14 - tq)]
+    assert r["state_amps"] == (1 << 14) + (1 << 16)
+    # round 1 of an optimizer iteration: the two gamma_0-shifted circuits of every QAOA instance
+    r1 = batch_emu.plan(b, 1)
+    assert r1["nsmall"] == 2 and r1["nwide"] == 4 and r1["nrounds"] == 1 + 6
+
+
+def test_batch_tile_width_does_not_change_results():
+    b = Batch()
+    e, w = workloads.erdos_renyi_graph(14, 0.5, 21)
+    b.add_qaoa(14, e, w, 2, [0.3, 1.1], [0.7, 0.2])
+    terms = workloads.random_pauli_hamiltonian(14, 8, 4, 22)
+    b.add_vqe(14, 1, terms, list(np.random.default_rng(1).uniform(-1, 1, 14)))
+    got = {}
+    for tq in (7, 12, 13):
+        if tq < 9:
+            with pytest.raises(Exception):
+                b.set_tile(tq)
+            continue
+        b.set_tile(tq)
+        got[tq] = batch_emu.evaluate(b)[0]
+    assert np.allclose(got[12], got[13], rtol=0, atol=1e-5 * (1 + sum(w)))
