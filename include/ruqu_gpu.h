@@ -262,13 +262,15 @@ int rq_batch_get_tile(const rq_batch* b);
 /* Host-only inspection (no device call; no reference counterpart): the launch
  * plan of one round (round < 0: the plain evaluation; else round r of an
  * optimizer iteration) as the device receives it.  info[8] = {nsmall, nwide,
- * ntiles, nlayers, nrounds, nouts, tq, 0}; info64[2] = {amplitudes of the
- * round's wide states, cut-table floats}; small/wide items are 16-byte
+ * ntiles, npasses, nrounds, nouts, tq, nvtiles}; info64[2] = {amplitudes of
+ * the round's wide states, cut-table floats}; small/wide items are 16-byte
  * records; wtile0/wstate: first tile and state offset per wide item; wmap:
- * wide item of each tile; layer_grid: tiles launched at each layer.  Null
- * outputs are skipped (call once for the counts). */
+ * wide item of each tile; pass_grid: tiles launched at each pass; vmap: the
+ * round's tiles of VQE items.  Null outputs are skipped (call once for the
+ * counts). */
 int rq_batch_plan_export(rq_batch* b, int round, int32_t* info, int64_t* info64, void* small_items,
-                         void* wide_items, int32_t* wtile0, int64_t* wstate, int32_t* wmap, int32_t* layer_grid);
+                         void* wide_items, int32_t* wtile0, int64_t* wstate, int32_t* wmap, int32_t* pass_grid,
+                         int32_t* vmap);
 /* Host-only: instance (48 B), edge (16 B), Pauli-term (32 B) and parameter
  * records in device layout; counts[4] = {instances, edges, terms, params}. */
 int rq_batch_data_export(rq_batch* b, void* insts, void* edges, void* terms, double* params, size_t* counts);

@@ -28,14 +28,12 @@ namespace rq {
 __global__ void k_batch_eval(const BInst* insts, const BItem* items, int nitems, unsigned* counter,
                              const int32_t* done, const BEdge* edges, const BTerm* terms, const double* params,
                              double* energy, uint32_t* amax);
-__global__ void k_wide_a(const BInst* insts, const BItem* witems, const int32_t* wmap, const int32_t* wtile0,
-                         const int64_t* wstate, const int32_t* done, const double* params, const float* ctab,
-                         float2* states, int tq, int layer);
-__global__ void k_wide_b(const BInst* insts, const BItem* witems, const int32_t* wmap, const int32_t* wtile0,
-                         const int64_t* wstate, const int32_t* done, const double* params, float2* states, int tq,
-                         int layer);
+template <int TQ>
+__global__ void k_wide_pass(const BInst* insts, const BItem* witems, const int32_t* wmap, const int32_t* wtile0,
+                            const int64_t* wstate, const int32_t* done, const double* params, const float* ctab,
+                            float2* states, int pass, BPart* parts);
 __global__ void k_wide_energy(const BInst* insts, const BItem* witems, const int32_t* wmap, const int32_t* wtile0,
-                              const int64_t* wstate, const int32_t* done, const BTerm* terms, const float* ctab,
+                              const int64_t* wstate, const int32_t* vmap, const int32_t* done, const BTerm* terms,
                               const float2* states, int tq, BPart* parts);
 __global__ void k_wide_fold(const BInst* insts, const BItem* witems, const int32_t* wtile0, const int32_t* done,
                             const BPart* parts, int tq, double* energy, uint32_t* amax);
@@ -141,7 +139,8 @@ struct RoundPlan {
   int small_off = 0, nsmall = 0;  // into all_small
   int wide_off = 0, nwide = 0;    // into all_wide / all_wtile0 / all_wstate
   int tile_off = 0, ntiles = 0;   // into all_wmap
-  std::vector<int> layer_grid;    // tiles of the items with more than l layers
+  std::vector<int> layer_grid;    // pass s: tiles of the items with more than s passes (p + 1)
+  int vmap_off = 0, nvtiles = 0;  // into all_vmap: the round's VQE tiles
   int64_t state_amps = 0;         // HBM amplitudes of this round's wide states
 };
 
@@ -167,12 +166,13 @@ struct Batch {
   std::vector<RoundPlan> rounds;
   RoundPlan base;
   std::vector<BItem> all_small, all_wide;
-  std::vector<int32_t> all_wtile0, all_wmap;
+  std::vector<int32_t> all_wtile0, all_wmap, all_vmap;
   std::vector<int64_t> all_wstate;
   BItem* d_small = nullptr;
   BItem* d_wide = nullptr;
   int32_t* d_wtile0 = nullptr;
   int32_t* d_wmap = nullptr;
+  int32_t* d_vmap = nullptr;
   int64_t* d_wstate = nullptr;
   float2* d_states = nullptr;
   int64_t states_cap = 0;  // amplitudes
@@ -195,6 +195,9 @@ struct Batch {
 };
 
 namespace {
+
+// Dynamic LDS of k_wide_pass: 2^tq amplitudes plus one pad slot per 2^(tq-8).
+size_t wide_lds_bytes(int tq) { return ((size_t(1) << tq) + (size_t(1) << 8)) * sizeof(float2); }
 
 double item_cost(const BInst& in) {
   double amps = std::ldexp(1.0, in.n);
@@ -223,18 +226,23 @@ RoundPlan plan_round(Batch& b, std::vector<BItem> v) {
   int64_t amps = 0;
   int maxl = 0;
   for (int j = 0; j < rp.nwide; ++j) maxl = std::max(maxl, b.insts[wide[j].inst].p);
-  rp.layer_grid.assign(maxl, 0);
+  rp.layer_grid.assign(rp.nwide ? maxl + 1 : 0, 0);
+  rp.vmap_off = (int)b.all_vmap.size();
   for (int j = 0; j < rp.nwide; ++j) {
     const BInst& in = b.insts[wide[j].inst];
     const int nt = 1 << (in.n - b.tq);
     b.all_wide.push_back(wide[j]);
     b.all_wtile0.push_back(tiles);
     b.all_wstate.push_back(amps);
-    for (int t = 0; t < nt; ++t) b.all_wmap.push_back(j);
+    for (int t = 0; t < nt; ++t) {
+      b.all_wmap.push_back(j);
+      if (in.kind == BK_VQE) b.all_vmap.push_back(tiles + t);
+    }
     tiles += nt;
     amps += (int64_t)1 << in.n;
-    for (int l = 0; l < in.p; ++l) rp.layer_grid[l] = tiles;  // sorted by p: a prefix
+    for (int l = 0; l <= in.p; ++l) rp.layer_grid[l] = tiles;  // sorted by p: a prefix
   }
+  rp.nvtiles = (int)b.all_vmap.size() - rp.vmap_off;
   rp.ntiles = tiles;
   rp.state_amps = amps;
   return rp;
@@ -291,6 +299,7 @@ void build_plan(Batch& b) {
   b.all_wide.clear();
   b.all_wtile0.clear();
   b.all_wmap.clear();
+  b.all_vmap.clear();
   b.all_wstate.clear();
   b.rounds.clear();
   for (int r = 0; r < b.nrounds; ++r) b.rounds.push_back(plan_round(b, rounds[r]));
@@ -311,11 +320,13 @@ int upload_plan(Batch& b) {
   free_dev(b.d_wide);
   free_dev(b.d_wtile0);
   free_dev(b.d_wmap);
+  free_dev(b.d_vmap);
   free_dev(b.d_wstate);
   int rc = upload(&b.d_small, b.all_small);
   if (!rc) rc = upload(&b.d_wide, b.all_wide);
   if (!rc) rc = upload(&b.d_wtile0, b.all_wtile0);
   if (!rc) rc = upload(&b.d_wmap, b.all_wmap);
+  if (!rc) rc = upload(&b.d_vmap, b.all_vmap);
   if (!rc) rc = upload(&b.d_wstate, b.all_wstate);
   if (rc) return rc;
   int64_t amps = b.base.state_amps;
@@ -354,10 +365,8 @@ int finalize(Batch& b) {
   RQ_HIP(hipStreamCreateWithFlags(&b.st, hipStreamNonBlocking));
   RQ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_eval), hipFuncAttributeMaxDynamicSharedMemorySize,
                              (1 << BATCH_LDS_QUBITS) * 8));
-  RQ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wide_a), hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (1 << BATCH_MAX_TILE_QUBITS) * 8));
-  RQ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wide_b), hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (1 << BATCH_MAX_TILE_QUBITS) * 8));
+  RQ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wide_pass<13>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, wide_lds_bytes(13)));
   const int ninst = (int)b.insts.size();
   int rc = upload(&b.d_insts, b.insts);
   if (!rc) rc = upload(&b.d_edges, b.edges);
@@ -406,16 +415,25 @@ void enqueue_round(Batch& b, const RoundPlan& rp, unsigned* counter, const int32
   const int32_t* wm = b.d_wmap + rp.tile_off;
   const int32_t* w0 = b.d_wtile0 + rp.wide_off;
   const int64_t* ws = b.d_wstate + rp.wide_off;
-  const size_t lds = (size_t(1) << b.tq) * sizeof(float2);
-  for (int l = 0; l < (int)rp.layer_grid.size(); ++l) {
-    const dim3 g(rp.layer_grid[l]);
-    hipLaunchKernelGGL(k_wide_a, g, dim3(BATCH_THREADS), lds, b.st, b.d_insts, wi, wm, w0, ws, done, b.d_params,
-                       b.d_ctab, b.d_states, b.tq, l);
-    hipLaunchKernelGGL(k_wide_b, g, dim3(BATCH_THREADS), lds, b.st, b.d_insts, wi, wm, w0, ws, done, b.d_params,
-                       b.d_states, b.tq, l);
+  const size_t lds = wide_lds_bytes(b.tq);
+  for (int ps = 0; ps < (int)rp.layer_grid.size(); ++ps) {
+    const dim3 g(rp.layer_grid[ps]), t(BATCH_THREADS);
+    switch (b.tq) {
+      case 9: hipLaunchKernelGGL(k_wide_pass<9>, g, t, lds, b.st, b.d_insts, wi, wm, w0, ws, done, b.d_params,
+                                 b.d_ctab, b.d_states, ps, b.d_parts); break;
+      case 10: hipLaunchKernelGGL(k_wide_pass<10>, g, t, lds, b.st, b.d_insts, wi, wm, w0, ws, done, b.d_params,
+                                  b.d_ctab, b.d_states, ps, b.d_parts); break;
+      case 11: hipLaunchKernelGGL(k_wide_pass<11>, g, t, lds, b.st, b.d_insts, wi, wm, w0, ws, done, b.d_params,
+                                  b.d_ctab, b.d_states, ps, b.d_parts); break;
+      case 12: hipLaunchKernelGGL(k_wide_pass<12>, g, t, lds, b.st, b.d_insts, wi, wm, w0, ws, done, b.d_params,
+                                  b.d_ctab, b.d_states, ps, b.d_parts); break;
+      default: hipLaunchKernelGGL(k_wide_pass<13>, g, t, lds, b.st, b.d_insts, wi, wm, w0, ws, done, b.d_params,
+                                  b.d_ctab, b.d_states, ps, b.d_parts); break;
+    }
   }
-  hipLaunchKernelGGL(k_wide_energy, dim3(rp.ntiles), dim3(BATCH_THREADS), 0, b.st, b.d_insts, wi, wm, w0, ws, done,
-                     b.d_terms, b.d_ctab, b.d_states, b.tq, b.d_parts);
+  if (rp.nvtiles)
+    hipLaunchKernelGGL(k_wide_energy, dim3(rp.nvtiles), dim3(BATCH_THREADS), 0, b.st, b.d_insts, wi, wm, w0, ws,
+                       b.d_vmap + rp.vmap_off, done, b.d_terms, b.d_states, b.tq, b.d_parts);
   hipLaunchKernelGGL(k_wide_fold, dim3(rp.nwide), dim3(BATCH_THREADS), 0, b.st, b.d_insts, wi, w0, done, b.d_parts,
                      b.tq, b.d_energy, b.d_amax);
 }
@@ -440,7 +458,7 @@ void free_batch(Batch& b) {
   if (b.st) (void)hipStreamSynchronize(b.st);
   if (b.exec) (void)hipGraphExecDestroy(b.exec);
   void* ptrs[] = {b.d_insts,  b.d_edges,  b.d_terms,  b.d_params, b.d_ctab,     b.d_small,   b.d_wide,
-                  b.d_wtile0, b.d_wmap,   b.d_wstate, b.d_states, b.d_parts,    b.d_energy,  b.d_amax,
+                  b.d_wtile0, b.d_wmap,   b.d_vmap,   b.d_wstate, b.d_states, b.d_parts,   b.d_energy,  b.d_amax,
                   b.d_counters, b.d_out0, b.d_outr,   b.o.iter,   b.o.hlen,     b.o.conv,    b.o.done,
                   b.o.hist,   b.o.best,   b.o.best_idx, b.d_best_params, b.d_zero_done};
   for (void* p : ptrs)
@@ -691,11 +709,12 @@ int rq_batch_get_tile(const rq_batch* h) {
 
 // Host-only (no device call): the launch plan of one round (round < 0: the
 // plain evaluation, else round r of an optimizer iteration), as the device
-// receives it.  info[0..7] = nsmall, nwide, ntiles, nlayers, nrounds, nouts,
-// tq, 0; info64[0..1] = HBM amplitudes of the round's wide states, cut-table
-// floats.  Null arrays are skipped (call once for the counts).
+// receives it.  info[0..7] = nsmall, nwide, ntiles, npasses, nrounds, nouts,
+// tq, nvtiles; info64[0..1] = HBM amplitudes of the round's wide states,
+// cut-table floats.  Null arrays are skipped (call once for the counts).
 int rq_batch_plan_export(rq_batch* h, int round, int32_t* info, int64_t* info64, void* small_items,
-                         void* wide_items, int32_t* wtile0, int64_t* wstate, int32_t* wmap, int32_t* layer_grid) {
+                         void* wide_items, int32_t* wtile0, int64_t* wstate, int32_t* wmap, int32_t* pass_grid,
+                         int32_t* vmap) {
   Batch* b = (Batch*)h;
   if (!b || b->insts.empty()) return fail(RQ_ERR_ARGUMENT, "empty batch");
   if (!b->planned) build_plan(*b);
@@ -703,7 +722,7 @@ int rq_batch_plan_export(rq_batch* h, int round, int32_t* info, int64_t* info64,
   const RoundPlan& rp = round < 0 ? b->base : b->rounds[round];
   if (info) {
     const int32_t v[8] = {rp.nsmall, rp.nwide, rp.ntiles, (int32_t)rp.layer_grid.size(), b->nrounds, b->nouts,
-                          b->tq, 0};
+                          b->tq, rp.nvtiles};
     std::memcpy(info, v, sizeof(v));
   }
   if (info64) {
@@ -715,7 +734,8 @@ int rq_batch_plan_export(rq_batch* h, int round, int32_t* info, int64_t* info64,
   if (wtile0) std::memcpy(wtile0, b->all_wtile0.data() + rp.wide_off, rp.nwide * sizeof(int32_t));
   if (wstate) std::memcpy(wstate, b->all_wstate.data() + rp.wide_off, rp.nwide * sizeof(int64_t));
   if (wmap) std::memcpy(wmap, b->all_wmap.data() + rp.tile_off, rp.ntiles * sizeof(int32_t));
-  if (layer_grid) std::copy(rp.layer_grid.begin(), rp.layer_grid.end(), layer_grid);
+  if (pass_grid) std::copy(rp.layer_grid.begin(), rp.layer_grid.end(), pass_grid);
+  if (vmap) std::memcpy(vmap, b->all_vmap.data() + rp.vmap_off, rp.nvtiles * sizeof(int32_t));
   return RQ_OK;
 }
 

@@ -7,14 +7,11 @@
 //            takes the next work item from a device counter (items sorted
 //            largest first, so uneven sizes share one launch without tails).
 //   n >  13 ("wide"): every state of the round has its own region in HBM and
-//            is spread over many workgroups, one per LDS tile of 2^tq
-//            amplitudes.  Each layer is two launches over all tiles of all
-//            wide items: k_wide_a (tiles of the low tq qubits: cost phase,
-//            rotations on those qubits) and k_wide_b (tiles holding the high
-//            n-tq qubits plus the lowest 2tq-n qubits as contiguous runs:
-//            rotations on the high qubits, the CZ chain).  k_wide_energy
-//            reduces every tile to an FP64 partial and k_wide_fold folds each
-//            item's partials in tile order (deterministic).
+//            is spread over many workgroups, one per tile of 2^tq amplitudes;
+//            k_wide_pass makes p + 1 passes for p layers (see below), the
+//            last QAOA pass yields per-tile FP64 energy partials, VQE energies
+//            come from k_wide_energy, and k_wide_fold folds each item's
+//            partials in tile order (deterministic).
 //
 // QAOA (qaoa.rs:125-153 circuit): |+>^n, then per layer the cost phase
 // prod_e Rzz(2 gamma w_e) = exp(i gamma (2 C(i) - W)) per amplitude (C(i) the
@@ -289,120 +286,254 @@ __global__ __launch_bounds__(B) void k_batch_eval(const BInst* __restrict__ inst
 }
 
 // ------------------------------------------------------------ wide items ----
-// Block b handles tile (b - wtile0[j]) of wide item j = wmap[b]; its state
-// starts at states + wstate[j].  All indices come from blockIdx (uniform).
+// A wide state of n qubits is cut into tiles of 2^tq amplitudes (h = n - tq
+// high qubits, lowb = tq - h) in one of two tilings:
+//   A: tile t = global bits tq..n-1; tile bits 0..tq-1 = global bits 0..tq-1.
+//   B: tile t = global bits lowb..tq-1; tile bits 0..lowb-1 = global bits
+//      0..lowb-1 and tile bits lowb..tq-1 = global bits tq..n-1.
+// A tiling rotates its "own" qubits: A the low tq, B the high h.  Pass s uses
+// tiling A for even s and B for odd s:
+//   QAOA, p layers, passes 0..p: pass s >= 1 first completes layer s-1 (Rx on
+//   its own qubits); for s < p it then applies the cost phase of layer s (a
+//   diagonal, valid in either tiling) and Rx of layer s on its own qubits.
+//   Pass p ends with the tile's energy partial and argmax instead of a store.
+//   VQE, L layers, passes 0..L: pass s >= 1 completes layer s-1 (Ry on its
+//   own qubits, then the CZ chain); for s < L it then applies Ry of layer s
+//   on its own qubits.
+// So p layers cost p + 1 passes over the state.  Inside a pass each thread
+// holds 2^(tq-8) amplitudes in registers: those whose tile indices differ in
+// a window of tq-8 tile bits; rotations on the window's bits are register
+// butterflies, and the window moves through an LDS transpose.
 
-__global__ __launch_bounds__(B) void k_wide_a(const BInst* __restrict__ insts, const BItem* __restrict__ witems,
-                                              const int32_t* __restrict__ wmap, const int32_t* __restrict__ wtile0,
-                                              const int64_t* __restrict__ wstate, const int32_t* __restrict__ done,
-                                              const double* __restrict__ params, const float* __restrict__ ctab,
-                                              float2* __restrict__ states, int tq, int layer) {
-  extern __shared__ float2 s[];
-  const int j = wmap[blockIdx.x];
-  const BItem it = witems[j];
-  const BInst in = insts[it.inst];
-  if (done[it.inst]) return;
-  const uint32_t T = 1u << tq, base = (uint32_t)(blockIdx.x - wtile0[j]) << tq;
-  float2* __restrict__ g = states + wstate[j];
-  if (in.kind == BK_QAOA) {
-    const double gamma = param_of(params, in, it, layer), beta = param_of(params, in, it, in.p + layer);
-    const float* __restrict__ ct = ctab + in.ctab_off;
-    const float c0 = plus_amp(in.n);
-    float c, sn;
-    half_angle(beta, &c, &sn);
-    for (uint32_t i = threadIdx.x; i < T; i += B) {
-      const float2 a = layer == 0 ? make_float2(c0, 0.0f) : g[base + i];
-      s[i] = phase_mul(a, gamma, (double)ct[base + i], in.wsum);
-    }
-    __syncthreads();
-    for (int q = 0; q < tq; ++q) lds_rx(s, tq, q, c, sn);
-  } else {
-    for (uint32_t i = threadIdx.x; i < T; i += B)
-      s[i] = layer == 0 ? make_float2(base + i == 0 ? 1.0f : 0.0f, 0.0f) : g[base + i];
-    __syncthreads();
-    for (int q = 0; q < tq; ++q) {
-      float c, sn;
-      half_angle(param_of(params, in, it, layer * in.n + q) / 2.0, &c, &sn);
-      lds_ry(s, tq, q, c, sn);
-    }
-  }
-  for (uint32_t i = threadIdx.x; i < T; i += B) g[base + i] = s[i];
+// LDS slot of tile index k: one pad slot every 2^r entries (bank spread when
+// a thread's registers are contiguous tile indices).
+__device__ __forceinline__ uint32_t lds_slot(uint32_t k, int r) { return k + (k >> r); }
+
+// Tile index of register m of thread `tid` for a window starting at bit lo.
+template <int R>
+__device__ __forceinline__ uint32_t win_index(uint32_t tid, int lo, int m) {
+  return (tid & ((1u << lo) - 1u)) | ((uint32_t)m << lo) | ((tid >> lo) << (lo + R));
 }
 
-__global__ __launch_bounds__(B) void k_wide_b(const BInst* __restrict__ insts, const BItem* __restrict__ witems,
-                                              const int32_t* __restrict__ wmap, const int32_t* __restrict__ wtile0,
-                                              const int64_t* __restrict__ wstate, const int32_t* __restrict__ done,
-                                              const double* __restrict__ params, float2* __restrict__ states, int tq,
-                                              int layer) {
-  extern __shared__ float2 s[];
-  const int j = wmap[blockIdx.x];
-  const BItem it = witems[j];
-  const BInst in = insts[it.inst];
-  if (done[it.inst]) return;
-  const int n = in.n, h = n - tq, lowb = tq - h;
-  const uint32_t T = 1u << tq, t = (uint32_t)(blockIdx.x - wtile0[j]), lom = (1u << lowb) - 1u;
-  float2* __restrict__ g = states + wstate[j];
-  for (uint32_t k = threadIdx.x; k < T; k += B) s[k] = g[(k & lom) | (t << lowb) | ((k >> lowb) << tq)];
+// Workgroups are dispatched to the 8 XCDs round-robin.  Within each chunk of
+// 32 workgroups this sends tiles 4x..4x+3 to XCD x, so the tiles of tiling B
+// that share 128-byte lines (lowb < 4) are read through one L2.
+__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t b, uint32_t grid) {
+  constexpr uint32_t G = 4, C = 8 * G;
+  if (b >= grid - grid % C) return b;
+  const uint32_t r = b % C;
+  return b - r + (r % 8) * G + r / 8;
+}
+
+template <int TQ>
+struct WideCtx {
+  static constexpr int R = TQ - 8, NR = 1 << R;
+  uint32_t t;
+  int n, lowb;
+  bool tb;  // tiling B
+  __device__ __forceinline__ uint32_t gidx(uint32_t k) const {
+    if (!tb) return (t << TQ) | k;
+    const uint32_t lom = (1u << lowb) - 1u;
+    return (k & lom) | (t << lowb) | ((k >> lowb) << TQ);
+  }
+  __device__ __forceinline__ int qubit_of(int bit) const { return tb ? TQ + bit - lowb : bit; }
+};
+
+template <int TQ>
+__device__ __forceinline__ void wide_transpose(float2 (&v)[1 << (TQ - 8)], int from, int to, float2* lds) {
+  constexpr int R = TQ - 8, NR = 1 << R;
+#pragma unroll
+  for (int m = 0; m < NR; ++m) lds[lds_slot(win_index<R>(threadIdx.x, from, m), R)] = v[m];
   __syncthreads();
-  if (in.kind == BK_QAOA) {
-    float c, sn;
-    half_angle(param_of(params, in, it, in.p + layer), &c, &sn);
-    for (int q = 0; q < h; ++q) lds_rx(s, tq, lowb + q, c, sn);
-    for (uint32_t k = threadIdx.x; k < T; k += B) g[(k & lom) | (t << lowb) | ((k >> lowb) << tq)] = s[k];
-  } else {
-    for (int q = 0; q < h; ++q) {
-      float c, sn;
-      half_angle(param_of(params, in, it, layer * n + tq + q) / 2.0, &c, &sn);
-      lds_ry(s, tq, lowb + q, c, sn);
-    }
-    for (uint32_t k = threadIdx.x; k < T; k += B) {
-      const uint32_t gi = (k & lom) | (t << lowb) | ((k >> lowb) << tq);
-      g[gi] = cz_chain(s[k], gi, n);
+#pragma unroll
+  for (int m = 0; m < NR; ++m) v[m] = lds[lds_slot(win_index<R>(threadIdx.x, to, m), R)];
+  __syncthreads();
+}
+
+// Rotations of one layer on the tile bits of window lo that are own bits not
+// yet rotated in this layer (done: mask of tile bits).  rx: Rx with
+// (c, s) = rc/rs[q] = (cos, sin) of half the angle; else Ry.
+template <int TQ>
+__device__ __forceinline__ void wide_rotate(float2 (&v)[1 << (TQ - 8)], int lo, int own_lo, uint32_t& done,
+                                            const WideCtx<TQ>& cx, bool rx, const float* rc, const float* rs) {
+  constexpr int R = TQ - 8, NR = 1 << R;
+#pragma unroll
+  for (int jb = 0; jb < R; ++jb) {
+    const int bit = lo + jb;
+    if (bit < own_lo || ((done >> bit) & 1u)) continue;
+    done |= 1u << bit;
+    const int q = cx.qubit_of(bit);
+    const float c = rc[q], sn = rs[q];
+#pragma unroll
+    for (int m = 0; m < NR; ++m) {
+      if (m & (1 << jb)) continue;
+      const float2 a = v[m], b = v[m | (1 << jb)];
+      if (rx) {
+        v[m] = make_float2(c * a.x + sn * b.y, c * a.y - sn * b.x);
+        v[m | (1 << jb)] = make_float2(sn * a.y + c * b.x, -sn * a.x + c * b.y);
+      } else {
+        v[m] = make_float2(c * a.x - sn * b.x, c * a.y - sn * b.y);
+        v[m | (1 << jb)] = make_float2(sn * a.x + c * b.x, sn * a.y + c * b.y);
+      }
     }
   }
 }
 
-// Per-tile partial: QAOA sum |a|^2 C(i) and the tile's argmax of |a|^2; VQE
-// sum_k c_k Re(conj(a_{i^x_k}) phase_k(i) a_i) over the tile's indices i.
+template <int TQ>
+__global__ __launch_bounds__(B) void k_wide_pass(const BInst* __restrict__ insts, const BItem* __restrict__ witems,
+                                                 const int32_t* __restrict__ wmap,
+                                                 const int32_t* __restrict__ wtile0,
+                                                 const int64_t* __restrict__ wstate,
+                                                 const int32_t* __restrict__ done_flags,
+                                                 const double* __restrict__ params, const float* __restrict__ ctab,
+                                                 float2* __restrict__ states, int pass, BPart* __restrict__ parts) {
+  constexpr int R = TQ - 8, NR = 1 << R;
+  extern __shared__ float2 lds[];  // 2^TQ + 2^(TQ-R) slots
+  __shared__ float rc[2][BATCH_MAX_QUBITS], rs[2][BATCH_MAX_QUBITS];
+  __shared__ Shared sh;
+  const uint32_t b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int j = wmap[b];
+  const BItem it = witems[j];
+  const BInst in = insts[it.inst];
+  if (done_flags[it.inst]) return;
+  const bool qaoa = in.kind == BK_QAOA;
+  const int L = in.p;
+  const bool l1 = pass >= 1, l2 = pass < L;
+  WideCtx<TQ> cx;
+  cx.n = in.n;
+  cx.lowb = 2 * TQ - in.n;
+  cx.tb = pass & 1;
+  cx.t = b - (uint32_t)wtile0[j];
+  const int own_lo = cx.tb ? cx.lowb : 0;
+  float2* __restrict__ g = states + wstate[j];
+  // rotation tables: [0] layer pass-1, [1] layer pass (half angles)
+  if (threadIdx.x < 2 * BATCH_MAX_QUBITS) {
+    const int sel = threadIdx.x / BATCH_MAX_QUBITS, q = threadIdx.x % BATCH_MAX_QUBITS;
+    const int layer = pass - 1 + sel;
+    float c = 1.0f, sn = 0.0f;
+    if (q < in.n && layer >= 0 && layer < L) {
+      if (qaoa) half_angle(param_of(params, in, it, L + layer), &c, &sn);
+      else half_angle(param_of(params, in, it, layer * in.n + q) / 2.0, &c, &sn);
+    }
+    rc[sel][q] = c;
+    rs[sel][q] = sn;
+  }
+  __syncthreads();
+  const int nw = (TQ - own_lo + R - 1) / R;  // windows covering the own bits, top first
+  float2 v[NR];
+  int lo;
+  if (pass == 0) {
+    lo = 0;
+    const float c0 = plus_amp(in.n);
+#pragma unroll
+    for (int m = 0; m < NR; ++m) {
+      const uint32_t gi = cx.gidx(win_index<R>(threadIdx.x, lo, m));
+      v[m] = qaoa ? make_float2(c0, 0.0f) : make_float2(gi == 0 ? 1.0f : 0.0f, 0.0f);
+    }
+  } else {
+    lo = TQ - R;
+#pragma unroll
+    for (int m = 0; m < NR; ++m) v[m] = g[cx.gidx(win_index<R>(threadIdx.x, lo, m))];
+  }
+  const float* __restrict__ ct = qaoa ? ctab + in.ctab_off : nullptr;
+  if (l1) {  // complete layer pass-1: windows top -> bottom
+    uint32_t dm = 0;
+    for (int w = 0; w < nw; ++w) {
+      const int wl = max(TQ - R * (w + 1), 0);
+      if (wl != lo) {
+        wide_transpose<TQ>(v, lo, wl, lds);
+        lo = wl;
+      }
+      wide_rotate<TQ>(v, lo, own_lo, dm, cx, qaoa, rc[0], rs[0]);
+    }
+    if (!qaoa) {
+#pragma unroll
+      for (int m = 0; m < NR; ++m) v[m] = cz_chain(v[m], cx.gidx(win_index<R>(threadIdx.x, lo, m)), in.n);
+    }
+  }
+  if (l2) {  // layer pass: (QAOA cost phase,) rotations bottom -> top
+    if (qaoa) {
+      const double gamma = param_of(params, in, it, pass);
+#pragma unroll
+      for (int m = 0; m < NR; ++m) {
+        const uint32_t gi = cx.gidx(win_index<R>(threadIdx.x, lo, m));
+        v[m] = phase_mul(v[m], gamma, (double)ct[gi], in.wsum);
+      }
+    }
+    uint32_t dm = 0;
+    for (int w = nw - 1; w >= 0; --w) {
+      const int wl = max(TQ - R * (w + 1), 0);
+      if (wl != lo) {
+        wide_transpose<TQ>(v, lo, wl, lds);
+        lo = wl;
+      }
+      wide_rotate<TQ>(v, lo, own_lo, dm, cx, qaoa, rc[1], rs[1]);
+    }
+  }
+  if (!l2 && qaoa) {  // last QAOA pass: energy partial and argmax of the tile
+    double acc = 0.0, bv = -1.0;
+    uint32_t bi = 0;
+#pragma unroll
+    for (int m = 0; m < NR; ++m) {
+      const uint32_t gi = cx.gidx(win_index<R>(threadIdx.x, lo, m));
+      const float2 a = v[m];
+      acc += ((double)a.x * a.x + (double)a.y * a.y) * (double)ct[gi];
+      better(bv, bi, (double)(a.x * a.x + a.y * a.y), gi);
+    }
+    const double e = block_sum_all(acc, sh.red);
+    block_argmax(bv, bi, sh.av, sh.ai);
+    const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    if (wid == 0) parts[b] = BPart{e, bv, bi, 0u};
+    return;
+  }
+#pragma unroll
+  for (int m = 0; m < NR; ++m) g[cx.gidx(win_index<R>(threadIdx.x, lo, m))] = v[m];
+}
+
+template __global__ void k_wide_pass<9>(const BInst*, const BItem*, const int32_t*, const int32_t*, const int64_t*,
+                                        const int32_t*, const double*, const float*, float2*, int, BPart*);
+template __global__ void k_wide_pass<10>(const BInst*, const BItem*, const int32_t*, const int32_t*, const int64_t*,
+                                         const int32_t*, const double*, const float*, float2*, int, BPart*);
+template __global__ void k_wide_pass<11>(const BInst*, const BItem*, const int32_t*, const int32_t*, const int64_t*,
+                                         const int32_t*, const double*, const float*, float2*, int, BPart*);
+template __global__ void k_wide_pass<12>(const BInst*, const BItem*, const int32_t*, const int32_t*, const int64_t*,
+                                         const int32_t*, const double*, const float*, float2*, int, BPart*);
+template __global__ void k_wide_pass<13>(const BInst*, const BItem*, const int32_t*, const int32_t*, const int64_t*,
+                                         const int32_t*, const double*, const float*, float2*, int, BPart*);
+
+// VQE tile partial sum_k c_k Re(conj(a_{i^x_k}) phase_k(i) a_i) over the
+// tile's indices i (natural order: tile t = indices t*2^tq ..).  Block b
+// handles round tile vmap[b].
 __global__ __launch_bounds__(B) void k_wide_energy(const BInst* __restrict__ insts, const BItem* __restrict__ witems,
                                                    const int32_t* __restrict__ wmap,
                                                    const int32_t* __restrict__ wtile0,
                                                    const int64_t* __restrict__ wstate,
+                                                   const int32_t* __restrict__ vmap,
                                                    const int32_t* __restrict__ done, const BTerm* __restrict__ terms,
-                                                   const float* __restrict__ ctab, const float2* __restrict__ states,
-                                                   int tq, BPart* __restrict__ parts) {
+                                                   const float2* __restrict__ states, int tq,
+                                                   BPart* __restrict__ parts) {
   __shared__ Shared sh;
-  const int j = wmap[blockIdx.x];
+  const int tile = vmap[blockIdx.x];
+  const int j = wmap[tile];
   const BItem it = witems[j];
   const BInst in = insts[it.inst];
   if (done[it.inst]) return;
-  const uint32_t T = 1u << tq, base = (uint32_t)(blockIdx.x - wtile0[j]) << tq;
+  const uint32_t T = 1u << tq, base = (uint32_t)(tile - wtile0[j]) << tq;
   const float2* __restrict__ g = states + wstate[j];
   const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
-  double acc = 0.0, bv = -1.0;
-  uint32_t bi = 0;
-  if (in.kind == BK_QAOA) {
-    const float* __restrict__ ct = ctab + in.ctab_off;
+  const BTerm* __restrict__ tt = terms + in.term_off;
+  double acc = 0.0;
+  for (int k = 0; k < in.nterms; ++k) {
+    const BTerm t = tt[k];
+    double a_k = 0.0;
     for (uint32_t i = threadIdx.x; i < T; i += B) {
-      const float2 a = g[base + i];
-      acc += ((double)a.x * a.x + (double)a.y * a.y) * (double)ct[base + i];
-      better(bv, bi, (double)(a.x * a.x + a.y * a.y), base + i);
+      const uint32_t gi = base + i;
+      a_k += pauli_contrib(g[gi], g[gi ^ (uint32_t)t.x], gi, t);
     }
-    block_argmax(bv, bi, sh.av, sh.ai);
-  } else {
-    const BTerm* __restrict__ tt = terms + in.term_off;
-    for (int k = 0; k < in.nterms; ++k) {
-      const BTerm t = tt[k];
-      double a_k = 0.0;
-      for (uint32_t i = threadIdx.x; i < T; i += B) {
-        const uint32_t gi = base + i;
-        a_k += pauli_contrib(g[gi], g[gi ^ (uint32_t)t.x], gi, t);
-      }
-      acc += t.c * a_k;
-    }
+    acc += t.c * a_k;
   }
   const double e = block_sum_all(acc, sh.red);
-  if (wid == 0) parts[blockIdx.x] = BPart{e, bv, bi, 0u};
+  if (wid == 0) parts[tile] = BPart{e, -1.0, 0u, 0u};
 }
 
 // One block per wide item: folds its tiles' partials (fixed assignment of

@@ -36,8 +36,7 @@ def _emu():
     vp, i = C.c_void_p, C.c_int
     lib.emu_cut_table.argtypes = [vp, i, vp, i]
     lib.emu_batch_eval.argtypes = [vp, vp, i] + [vp] * 6 + [i]
-    lib.emu_wide_a.argtypes = [i] + [vp] * 9 + [i, i]
-    lib.emu_wide_b.argtypes = [i] + [vp] * 8 + [i, i]
+    lib.emu_wide_pass.argtypes = [i] + [vp] * 9 + [i, i, vp]
     lib.emu_wide_energy.argtypes = [i] + [vp] * 9 + [i, vp]
     lib.emu_wide_fold.argtypes = [i] + [vp] * 5 + [i, vp, vp]
     return lib
@@ -59,21 +58,24 @@ def plan(batch, rnd=-1):
     """The launch plan of one round (-1: plain evaluation) as batch.cpp builds it."""
     lib = _native.lib()
     lib.rq_batch_plan_export.argtypes = [C.c_void_p, C.c_int, I32P, I64P, C.c_void_p, C.c_void_p, I32P, I64P, I32P,
-                                         I32P]
+                                         I32P, I32P]
     info = np.zeros(8, dtype=np.int32)
     info64 = np.zeros(2, dtype=np.int64)
     ip = lambda a: a.ctypes.data_as(I32P)  # noqa: E731
     lp = lambda a: a.ctypes.data_as(I64P)  # noqa: E731
-    assert lib.rq_batch_plan_export(batch._h, rnd, ip(info), lp(info64), None, None, None, None, None, None) == 0
-    nsmall, nwide, ntiles, nlayers = (int(x) for x in info[:4])
+    assert lib.rq_batch_plan_export(batch._h, rnd, ip(info), lp(info64), None, None, None, None, None, None,
+                                    None) == 0
+    nsmall, nwide, ntiles, npasses = (int(x) for x in info[:4])
+    nvtiles = int(info[7])
     r = dict(small=np.zeros(max(nsmall, 1), dtype=ITEM)[:nsmall], wide=np.zeros(max(nwide, 1), dtype=ITEM)[:nwide],
              wtile0=np.zeros(max(nwide, 1), dtype=np.int32), wstate=np.zeros(max(nwide, 1), dtype=np.int64),
-             wmap=np.zeros(max(ntiles, 1), dtype=np.int32), layer_grid=np.zeros(max(nlayers, 1), dtype=np.int32))
+             wmap=np.zeros(max(ntiles, 1), dtype=np.int32), pass_grid=np.zeros(max(npasses, 1), dtype=np.int32),
+             vmap=np.zeros(max(nvtiles, 1), dtype=np.int32))
     assert lib.rq_batch_plan_export(batch._h, rnd, None, None, _p(r["small"]), _p(r["wide"]), ip(r["wtile0"]),
-                                    lp(r["wstate"]), ip(r["wmap"]), ip(r["layer_grid"])) == 0
-    r.update(nsmall=nsmall, nwide=nwide, ntiles=ntiles, nlayers=nlayers, nrounds=int(info[4]), nouts=int(info[5]),
-             tq=int(info[6]), state_amps=int(info64[0]), ctab_total=int(info64[1]))
-    r["layer_grid"] = r["layer_grid"][:nlayers]
+                                    lp(r["wstate"]), ip(r["wmap"]), ip(r["pass_grid"]), ip(r["vmap"])) == 0
+    r.update(nsmall=nsmall, nwide=nwide, ntiles=ntiles, npasses=npasses, nvtiles=nvtiles, nrounds=int(info[4]),
+             nouts=int(info[5]), tq=int(info[6]), state_amps=int(info64[0]), ctab_total=int(info64[1]))
+    r["pass_grid"] = r["pass_grid"][:npasses]
     return r
 
 
@@ -100,10 +102,11 @@ def evaluate(batch, grid=3):
         states = np.zeros(max(r["state_amps"], 1), dtype=np.complex64)
         parts = np.zeros(r["ntiles"], dtype=PART)
         w = (_p(insts), _p(r["wide"]), _p(r["wmap"]), _p(r["wtile0"]), _p(r["wstate"]), _p(done))
-        for layer, g in enumerate(r["layer_grid"]):
-            lib.emu_wide_a(int(g), *w, _p(pa), _p(ctab), _p(states), r["tq"], layer)
-            lib.emu_wide_b(int(g), *w, _p(pa), _p(states), r["tq"], layer)
-        lib.emu_wide_energy(r["ntiles"], *w, _p(ta), _p(ctab), _p(states), r["tq"], _p(parts))
+        for ps, g in enumerate(r["pass_grid"]):
+            lib.emu_wide_pass(int(g), *w, _p(pa), _p(ctab), _p(states), r["tq"], ps, _p(parts))
+        if r["nvtiles"]:
+            lib.emu_wide_energy(r["nvtiles"], *w[:5], _p(r["vmap"]), _p(done), _p(ta), _p(states), r["tq"],
+                                _p(parts))
         lib.emu_wide_fold(r["nwide"], _p(insts), _p(r["wide"]), _p(r["wtile0"]), _p(done), _p(parts), r["tq"],
                           _p(energy), _p(amax))
     en = np.zeros(len(insts))

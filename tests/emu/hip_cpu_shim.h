@@ -1,17 +1,15 @@
-// Host-thread emulation of the HIP device features used by the batched
-// kernel: one std::thread per work-item of a block, a real barrier for
-// __syncthreads, per-wave barriers for readfirstlane, shuffles through a
-// per-block exchange array.  Lets the
-// kernel source run on the CPU for tests (no GPU involved).
+// Host emulation of the HIP device features used by the batched kernels (test
+// infrastructure): the 256 work-items of a block are cooperative contexts run
+// by one host thread and switched at barriers (block barrier for
+// __syncthreads, a 64-item barrier for readfirstlane), shuffles go through a
+// per-block exchange array, and blocks run one after another.  Lets the
+// kernel source run on the CPU without a GPU.
 #pragma once
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
 
-#include <atomic>
-#include <barrier>
 #include <cmath>
-#include <thread>
 #include <vector>
 
 struct float2 {
@@ -25,12 +23,15 @@ inline float2 make_float2(float x, float y) { return float2{x, y}; }
 struct Dim3 {
   unsigned x = 1, y = 1, z = 1;
 };
-extern thread_local Dim3 threadIdx, blockIdx;
-extern Dim3 blockDim, gridDim;
+extern Dim3 threadIdx, blockIdx, blockDim, gridDim;
 
 namespace emu {
-extern std::barrier<>* bar;
-extern std::barrier<>* wbar[4];  // one per 64-thread wave of the block
+struct Barrier {
+  int count = 0, target = 0;
+  unsigned gen = 0;
+};
+void wait(Barrier& b);  // yields to the other work-items until all arrived
+extern Barrier block_bar, wave_bar[4];
 extern int wx[4];
 extern std::vector<double> xchg;
 extern unsigned char* lds;
@@ -43,7 +44,7 @@ extern unsigned char* lds;
 #define __launch_bounds__(...)
 #define __restrict__
 
-inline void __syncthreads() { emu::bar->arrive_and_wait(); }
+inline void __syncthreads() { emu::wait(emu::block_bar); }
 
 inline double __shfl_down(double v, int off, int width) {
   unsigned t = threadIdx.x;
@@ -59,11 +60,13 @@ inline double __shfl_down(double v, int off, int width) {
 inline int __builtin_amdgcn_readfirstlane(int v) {
   const unsigned w = threadIdx.x / 64;
   if (threadIdx.x % 64 == 0) emu::wx[w] = v;
-  emu::wbar[w]->arrive_and_wait();
+  emu::wait(emu::wave_bar[w]);
   const int r = emu::wx[w];
-  emu::wbar[w]->arrive_and_wait();
+  emu::wait(emu::wave_bar[w]);
   return r;
 }
+inline int max(int a, int b) { return a > b ? a : b; }
+inline int min(int a, int b) { return a < b ? a : b; }
 inline int __popc(uint32_t x) { return __builtin_popcount(x); }
 inline int __popcll(uint64_t x) { return __builtin_popcountll(x); }
 inline unsigned atomicAdd(unsigned* p, unsigned v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }

@@ -67,7 +67,9 @@ def test_batch_plan_order_and_layout():
     assert list(r["wtile0"]) == [0, 1 << (14 - tq)]
     assert list(r["wstate"]) == [0, 1 << 14]
     assert r["ntiles"] == (1 << (14 - tq)) + (1 << (16 - tq)) and len(r["wmap"]) == r["ntiles"]
-    assert list(r["layer_grid"]) == [r["ntiles"], r["ntiles"], 1 << (14 - tq)]
+    # p + 1 passes per item: both items in passes 0-2, the 14-qubit p=3 item alone in pass 3
+    assert list(r["pass_grid"]) == [r["ntiles"], r["ntiles"], r["ntiles"], 1 << (14 - tq)]
+    assert r["nvtiles"] == 0
     assert r["state_amps"] == (1 << 14) + (1 << 16)
     # round 1 of an optimizer iteration: the two gamma_0-shifted circuits of every QAOA instance
     r1 = batch_emu.plan(b, 1)
@@ -81,11 +83,32 @@ def test_batch_tile_width_does_not_change_results():
     terms = workloads.random_pauli_hamiltonian(14, 8, 4, 22)
     b.add_vqe(14, 1, terms, list(np.random.default_rng(1).uniform(-1, 1, 14)))
     got = {}
-    for tq in (7, 12, 13):
+    for tq in (7, 11, 12, 13):
         if tq < 9:
             with pytest.raises(Exception):
                 b.set_tile(tq)
             continue
         b.set_tile(tq)
         got[tq] = batch_emu.evaluate(b)[0]
-    assert np.allclose(got[12], got[13], rtol=0, atol=1e-5 * (1 + sum(w)))
+    tol = 1e-5 * (1 + sum(w) + sum(abs(c) for c, _ in terms))
+    assert np.allclose(got[12], got[13], rtol=0, atol=tol) and np.allclose(got[11], got[12], rtol=0, atol=tol)
+
+
+def test_batch_wide_edge_tilings_on_cpu():
+    """Smallest tile (2^9) with 17 and 18 qubits: tiling B with 1 and 0 low
+    bits (every tile bit a high qubit), the XCD-swizzled tile order on 512
+    tiles, QAOA energies taken in the last pass of either tiling."""
+    b = Batch()
+    qc = []
+    for k, (n, p) in enumerate([(18, 1), (17, 2)]):
+        e, w = workloads.erdos_renyi_graph(n, 0.3, 500 + k)
+        rng = np.random.default_rng(70 + k)
+        qc.append((n, e, w, p, list(rng.uniform(0, math.pi, p)), list(rng.uniform(0, math.pi, p))))
+        b.add_qaoa(*qc[-1])
+    b.set_tile(9)
+    en, am = batch_emu.evaluate(b)
+    for k, (n, e, w, p, g, be) in enumerate(qc):
+        want, amps = orc.qaoa_cost(n, e, w, g, be, want_state=True)
+        assert abs(en[k] - want) <= 1e-5 * (1 + sum(w)), (n, en[k], want)
+        pr = np.abs(amps) ** 2
+        assert pr[am[k]] >= pr.max() * (1 - 1e-4)

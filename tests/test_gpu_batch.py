@@ -113,3 +113,41 @@ def test_batch_optimizer_follows_reference_loops(use_graph):
     assert np.max(np.abs(got["energy_history"] - hist)) < 1e-4
     assert np.max(np.abs(got["params"] - params)) < 1e-3
     assert abs(got["best"] - best) < 1e-4
+
+
+@pytest.mark.parametrize("tq", [9, 11, 12, 13])
+def test_batch_tile_widths_match_oracle(tq):
+    """Wide states at every tile width, including tiling B with 0-3 low bits
+    (n close to 2*tq: tiles sharing 128-byte lines, the XCD-swizzled order)."""
+    sizes = [(14, 3), (16, 1), (17, 2), (18, 1)] + ([] if tq == 9 else [(20, 2)])
+    b = Batch()
+    qc = []
+    for k, (n, p) in enumerate(sizes):
+        e, w = workloads.erdos_renyi_graph(n, 0.4, 600 + k)
+        rng = np.random.default_rng(80 + k)
+        qc.append((n, e, w, p, list(rng.uniform(0, math.pi, p)), list(rng.uniform(0, math.pi, p))))
+        b.add_qaoa(*qc[-1])
+    terms = workloads.random_pauli_hamiltonian(15, 25, 4, 700)
+    prm = list(np.random.default_rng(90).uniform(-1, 1, 15 * 3))
+    b.add_vqe(15, 3, terms, prm)
+    b.set_tile(tq)
+    en, amax = b.evaluate()
+    print("tile %d: evaluate %.3f ms" % (tq, b.last_ms()), flush=True)
+    for k, (n, e, w, p, g, be) in enumerate(qc):
+        want, amps = orc.qaoa_cost(n, e, w, g, be, want_state=True)
+        assert abs(en[k] - want) <= 1e-5 * (1 + sum(w)), (tq, n, en[k], want)
+        pr = np.abs(amps) ** 2
+        assert pr[amax[k]] >= pr.max() * (1 - 1e-4)
+    want = orc.energy(15, workloads.ry_cz_ansatz(15, 3, prm).to_array(), terms)
+    assert abs(en[-1] - want) <= 1e-5 * (1 + sum(abs(c) for c, _ in terms))
+
+
+def test_batch_24_qubits_tile_widths_agree():
+    e, w = workloads.erdos_renyi_graph(24, 0.3, 800)
+    b = Batch()
+    b.add_qaoa(24, e, w, 2, [0.4, 0.9], [0.3, 0.6])
+    got = {}
+    for tq in (12, 13):
+        b.set_tile(tq)
+        got[tq] = b.evaluate()
+    assert abs(got[12][0][0] - got[13][0][0]) <= 1e-5 * (1 + sum(w))
