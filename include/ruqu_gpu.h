@@ -304,6 +304,38 @@ int rq_time_1q(rq_state* s, uint32_t qubit, const rq_c64* m, int reps, double* a
  * write N) as the achievable-bandwidth reference; *avg_ms per launch. */
 int rq_time_copy(rq_state* s, int reps, double* avg_ms);
 
+/* ---- learned warm start (no reference counterpart) ---------------------
+ * A graph network predicting QAOA angles (gamma_1..8, beta_1..8) from a
+ * weighted graph and the depth p: 3 message-passing layers of width 256 with
+ * bfloat16 matrix-core products (FP32 accumulation), trained on the device
+ * by its own backward pass and Adam.  Its specification (parameter layout,
+ * features, loss) is the FP64 restatement in oracle/mpnn_oracle.py. */
+typedef struct rq_mpnn rq_mpnn;
+typedef struct {
+  int32_t ngraphs, nnodes;
+  int64_t nedges;
+  const int32_t* node_off; /* ngraphs+1: nodes of graph g are node_off[g] .. node_off[g+1]-1 */
+  const int32_t* row_ptr;  /* nnodes+1: incoming edges of node v (CSR by destination) */
+  const int32_t* col;      /* nedges: source node of each edge (both directions listed) */
+  const float* w;          /* nedges: edge weight */
+  const float* x;          /* nnodes x 32 node features (degree/16, weighted degree/8, 0...) */
+  const float* gfeat;      /* ngraphs x 32 graph features (onehot(p), n/22, edge density, 0...) */
+  const int32_t* p;        /* ngraphs: QAOA depth, 1..8 */
+} rq_mpnn_batch;
+size_t rq_mpnn_num_params(void);
+int rq_mpnn_create(rq_mpnn** out);
+void rq_mpnn_destroy(rq_mpnn* n);
+/* Flat FP32 parameters (layout of oracle/mpnn_oracle.py); set also resets Adam. */
+int rq_mpnn_set_params(rq_mpnn* n, const float* params);
+int rq_mpnn_get_params(const rq_mpnn* n, float* params);
+/* angles: ngraphs x 16 (gammas then betas; entries l >= p are unused). */
+int rq_mpnn_forward(rq_mpnn* n, const rq_mpnn_batch* b, float* angles);
+/* Masked-MSE loss against targets (ngraphs x 16) and its gradient (FP32,
+ * rq_mpnn_num_params() values; grad may be null). */
+int rq_mpnn_gradient(rq_mpnn* n, const rq_mpnn_batch* b, const float* targets, float* loss, float* grad);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the batch; loss before the step. */
+int rq_mpnn_train_step(rq_mpnn* n, const rq_mpnn_batch* b, const float* targets, float lr, float* loss);
+
 #ifdef __cplusplus
 }
 #endif
