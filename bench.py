@@ -12,7 +12,15 @@ circuit's original gate applications per second.  The line also carries:
                 HBM pass per gate), GB/s per target qubit;
   cpu_baseline  the reference's f32 CPU path (QuantumStateF32 scalar loops,
                 restated in the oracle), one host thread, on the first gates
-                of the same circuit.
+                of the same circuit;
+  qaoa          the batched QAOA part of the metric: held-out MaxCut
+                instances (G(n, 0.5), n ~ U{10..22}, U(0,1) weights,
+                p ~ U{1..8}) through the plain batched optimizer (run_qaoa's
+                rule, 16 iterations) and through the learned warm start
+                (network prediction, default start and per-depth constant,
+                each judged by one batched evaluation; plain fallback when
+                the expected cut is below 99 % of plain's): instances/s of
+                both and the fraction within 1 %.
 """
 import argparse
 import json
@@ -69,6 +77,10 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--qaoa", type=int, default=1024, help="held-out QAOA instances (0: skip)")
+    ap.add_argument("--no-f64-check", action="store_true", help="skip the f32-vs-f64 error of the circuit")
+    ap.add_argument("--no-dense", action="store_true", help="skip the dense 5-qubit block (matrix-core) run")
+    ap.add_argument("--qaoa-iters", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU share "
@@ -131,10 +143,80 @@ def main():
         copy_gbs = round(stage_bytes / (cavg.value * 1e-3) / 1e9, 1)
     log("per-gate GB/s %s, copy %s GB/s" % (per_gate, copy_gbs))
 
+    err = None
+    if prec == "c32" and not args.no_f64_check and nq <= 31:
+        log("f64 run of the same circuit for the f32 error ...")
+        a32 = st.state_vector()
+        st64 = QuantumState.new_with_seed(nq, args.seed, "c64")
+        rc = L.rq_run_circuit(st64.handle, ptr(arr), ngates, None, 0, None, 0, C.byref(nout), 1, C.byref(met))
+        if rc:
+            raise RuntimeError(L.rq_last_error().decode())
+        a64 = st64.state_vector()
+        del st64
+        mx, d2, n64, n32 = 0.0, 0.0, 0.0, 0.0
+        ip = 0j
+        ch = 1 << 24
+        for o in range(0, n, ch):
+            x = a32[o:o + ch].astype(np.complex128)
+            y = a64[o:o + ch]
+            d = x - y
+            mx = max(mx, float(np.max(np.abs(d))))
+            d2 += float(np.vdot(d, d).real)
+            n64 += float(np.vdot(y, y).real)
+            n32 += float(np.vdot(x, x).real)
+            ip += complex(np.vdot(y, x))
+        del a32, a64
+        err = {"max_abs": mx, "rel_l2": (d2 / n64) ** 0.5, "one_minus_fidelity": 1.0 - abs(ip) ** 2 / (n32 * n64),
+               "note": "complex-f32 fused run vs complex-f64 fused run of the same circuit on the device "
+                       "(the f64 path matches the FP64 CPU oracle to 1e-12)"}
+        log("f32 error: %s" % err)
+
+    dense = None
+    if prec == "c32" and not args.no_dense:
+        log("dense 5-qubit blocks on the matrix cores ...")
+        dms = []
+        for k in range(2):
+            rc = L.rq_run_circuit(st.handle, ptr(arr), ngates, None, 0, None, 0, C.byref(nout), 2, C.byref(met))
+            if rc:
+                raise RuntimeError(L.rq_last_error().decode())
+            dms.append(met.device_time_ms)
+        blocks = int(met.passes)
+        per_block_ms = min(dms) / blocks
+        flops = 256.0 * n  # real 64x64 GEMM row per 32 amplitudes: 8 * 32 flop per amplitude
+        dense = {"kernel": "k_dense5_f32", "blocks": blocks, "circuit_ms": round(min(dms), 3),
+                 "gate_apps_per_s": round(ngates / (min(dms) * 1e-3), 2),
+                 "mean_block_ms": round(per_block_ms, 4),
+                 "mfma_TFLOPs": round(flops / (per_block_ms * 1e-3) / 1e12, 2),
+                 "mfma_peak_TFLOPs": 157.3,
+                 "mfma_frac": round(flops / (per_block_ms * 1e-3) / 1e12 / 157.3, 4),
+                 "hbm_GBps": round(stage_bytes / (per_block_ms * 1e-3) / 1e9, 1),
+                 "note": "same circuit as dense 32x32 complex unitaries (v_mfma_f32_32x32x2_f32); "
+                         "the fused tile schedule above needs %d passes instead of %d" % (passes, blocks)}
+        log("dense: %s" % dense)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
         cpu = cpu_baseline(nq, arr, args.cpu_budget)
+
+    qaoa = None
+    params_path = os.path.join(ROOT, "ruqu_amd", "data", "warmstart_params.npy")
+    if args.qaoa > 0 and os.path.exists(params_path):
+        from ruqu_amd import warmstart
+        from ruqu_amd.mpnn import Network
+        del st  # free the 8 GiB state before the batched instances
+        log("qaoa: %d held-out instances, plain %d iterations vs warm start" % (args.qaoa, args.qaoa_iters))
+        insts = warmstart.held_out(args.qaoa)
+        warmstart.run_plain(insts[:32], 1)  # warm-up of the batched kernels
+        net = Network.load(params_path)
+        base = warmstart.plain_baseline(insts, args.qaoa_iters)
+        cands = [net, warmstart.DefaultInit(), warmstart.ConstantPredictor.load()]
+        r = warmstart.evaluate(cands, insts, args.qaoa_iters, 0, log=log, plain=base)
+        qaoa = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+        qaoa["workload"] = ("held-out MaxCut: G(n,0.5) n~U{10..22} U(0,1) weights p~U{1..8}, complex-f32 states, "
+                            "FP64 reductions")
+    elif args.qaoa > 0:
+        log("qaoa part skipped: no trained parameters at %s" % params_path)
 
     value = ngates * args.steps / el
     line = {
@@ -160,6 +242,9 @@ def main():
         "per_gate_1q_GBps": per_gate,
         "copy_GBps": copy_gbs,
         "cpu_baseline": cpu,
+        "f32_error_vs_f64": err,
+        "dense_blocks": dense,
+        "qaoa": qaoa,
     }
     print(json.dumps(line), flush=True)
 

@@ -144,7 +144,9 @@ int rq_apply_2q(rq_state* s, uint32_t q1, uint32_t q2, const rq_c64* m);
  * order; measurements receive up to *nout_cap outcomes, *nout the count.
  * fusion=0: one exact kernel per gate (the reference's arithmetic, without
  * FMA contraction).  fusion=1: the fused schedule (gates folded into
- * LDS-resident tiles / dense blocks; rounding differs at the 1e-15 level). */
+ * LDS/register-resident tiles; rounding differs at the 1e-15 level).
+ * fusion=2: dense 5-qubit blocks, each a 32x32 complex unitary applied on
+ * the matrix cores (complex-f32 states of >= 10 qubits; others use 1). */
 int rq_run_circuit(rq_state* s, const rq_gate* gates, size_t ngates, const double* draws, size_t ndraws,
                    rq_measurement* measurements, size_t nout_cap, size_t* nout, int fusion,
                    rq_metrics* metrics);
@@ -247,12 +249,20 @@ int rq_batch_evaluate(rq_batch* b, double* energies, uint64_t* argmax);
 int rq_batch_optimize(rq_batch* b, uint32_t max_iter, double qaoa_lr, double vqe_lr, double vqe_threshold,
                       int use_graph, double* device_ms);
 /* Results of one instance: best energy (QAOA max / VQE min), QAOA best
- * bitstring index, history length, converged flag, final parameters, VQE best
- * parameters, energy history. */
+ * bitstring index, history length, converged flag, final parameters, the
+ * parameters of the best energy, energy history. */
 int rq_batch_result(rq_batch* b, uint32_t id, double* best, uint64_t* best_index, uint32_t* iterations,
                     int* converged, double* params, double* best_params, double* history);
+/* rq_batch_result for every instance at once; params / best_params hold all
+ * instances' parameters back to back in instance order. */
+int rq_batch_results(rq_batch* b, double* best, uint64_t* best_index, uint32_t* iterations, int* converged,
+                     double* params, double* best_params);
 int rq_batch_set_params(rq_batch* b, uint32_t id, const double* params);
 uint32_t rq_batch_size(const rq_batch* b);
+/* Exact MaxCut value of every QAOA instance (device brute force over all 2^n
+ * bitstrings, FP64 weights; 0 for VQE instances): the denominator of
+ * approximation ratios (no reference counterpart). */
+int rq_batch_maxcut(rq_batch* b, double* out);
 /* HIP-event time of the last evaluate/optimize call. */
 double rq_batch_last_ms(const rq_batch* b);
 /* LDS tile of the states above 13 qubits: 2^tq amplitudes per workgroup,

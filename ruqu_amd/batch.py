@@ -30,6 +30,8 @@ _native.register("rq_batch_set_params", C.c_int, [C.c_void_p, C.c_uint32, C.c_vo
 _native.register("rq_batch_size", C.c_uint32, [C.c_void_p])
 _native.register("rq_batch_last_ms", C.c_double, [C.c_void_p])
 _native.register("rq_batch_set_tile", C.c_int, [C.c_void_p, C.c_int])
+_native.register("rq_batch_maxcut", C.c_int, [C.c_void_p, C.c_void_p])
+_native.register("rq_batch_results", C.c_int, [C.c_void_p] + [C.c_void_p] * 6)
 _native.register("rq_batch_get_tile", C.c_int, [C.c_void_p])
 
 
@@ -114,17 +116,37 @@ class Batch:
                                     ptr(params), ptr(bparams), ptr(hist)))
         out = {"best": best.value, "iterations": iters.value, "converged": bool(conv.value), "params": params,
                "energy_history": hist[: iters.value].copy()}
+        out.update(best_params=bparams)
         if self._kinds[i] == "qaoa":
             p = self._nparams[i] // 2
             out.update(best_index=int(bidx.value), optimal_gammas=params[:p], optimal_betas=params[p:],
+                       best_gammas=bparams[:p], best_betas=bparams[p:],
                        best_bitstring=[bool((bidx.value >> v) & 1) for v in range(self._n[i])])
-        else:
-            out.update(best_params=bparams)
         return out
 
     def set_params(self, i, params):
         a = np.ascontiguousarray(params, dtype=np.float64)
         check(lib().rq_batch_set_params(self._h, i, ptr(a)))
+
+    def maxcut(self):
+        """Exact MaxCut value of every QAOA instance (device brute force)."""
+        out = np.zeros(len(self), dtype=np.float64)
+        check(lib().rq_batch_maxcut(self._h, ptr(out)))
+        return out
+
+    def results(self):
+        """Bulk results: dict of arrays (best, best_index, iterations,
+        converged) and per-instance lists params / best_params."""
+        n = len(self)
+        best, bidx = np.zeros(n), np.zeros(n, dtype=np.uint64)
+        iters, conv = np.zeros(n, dtype=np.uint32), np.zeros(n, dtype=np.int32)
+        tot = int(sum(self._nparams))
+        prm, bprm = np.zeros(max(tot, 1)), np.zeros(max(tot, 1))
+        check(lib().rq_batch_results(self._h, ptr(best), ptr(bidx), ptr(iters), ptr(conv), ptr(prm), ptr(bprm)))
+        offs = np.concatenate([[0], np.cumsum(self._nparams)]).astype(np.int64)
+        return {"best": best, "best_index": bidx, "iterations": iters, "converged": conv.astype(bool),
+                "params": [prm[offs[i]:offs[i + 1]] for i in range(n)],
+                "best_params": [bprm[offs[i]:offs[i + 1]] for i in range(n)]}
 
     def set_tile(self, tq):
         """LDS tile (2^tq amplitudes per workgroup) of the states above 13 qubits."""

@@ -38,6 +38,7 @@ __global__ void k_wide_energy(const BInst* insts, const BItem* witems, const int
 __global__ void k_wide_fold(const BInst* insts, const BItem* witems, const int32_t* wtile0, const int32_t* done,
                             const BPart* parts, int tq, double* energy, uint32_t* amax);
 __global__ void k_cut_table(float* ctab, int n, const BEdge* e, int ne);
+__global__ void k_maxcut(int n, const BEdge* e, int ne, unsigned long long* out);
 __global__ void k_zero_u32(unsigned* p, int n);
 
 namespace {
@@ -76,9 +77,10 @@ __global__ void k_update0(const BInst* insts, int ninst, const int32_t* out0, co
   h[it] = e;
   o.hlen[k] = it + 1;
   if (in.kind == BK_QAOA) {
-    if (e > o.best[k]) {
+    if (e > o.best[k]) {  // qaoa.rs:240-256; the angles of the best energy are kept as well
       o.best[k] = e;
       o.best_idx[k] = amax[out0[k]];
+      for (int j = 0; j < in.nparams; ++j) o.best_params[in.param_off + j] = params[in.param_off + j];
     }
     if (it > 0 && fabs(e - h[it - 1]) < 1e-6) {
       o.conv[k] = 1;
@@ -662,7 +664,7 @@ int rq_batch_result(rq_batch* h, uint32_t id, double* best, uint64_t* best_index
   if (converged) *converged = cv;
   if (params)
     RQ_HIP(hipMemcpy(params, b->d_params + in.param_off, in.nparams * sizeof(double), hipMemcpyDeviceToHost));
-  if (best_params && in.kind == BK_VQE)
+  if (best_params && b->o.hist)
     RQ_HIP(hipMemcpy(best_params, b->d_best_params + in.param_off, in.nparams * sizeof(double),
                      hipMemcpyDeviceToHost));
   return RQ_OK;
@@ -755,6 +757,62 @@ int rq_batch_data_export(rq_batch* h, void* insts, void* edges, void* terms, dou
   if (edges) std::memcpy(edges, b->edges.data(), b->edges.size() * sizeof(BEdge));
   if (terms) std::memcpy(terms, b->terms.data(), b->terms.size() * sizeof(BTerm));
   if (params) std::memcpy(params, b->params.data(), b->params.size() * sizeof(double));
+  return RQ_OK;
+}
+
+// All instances at once after rq_batch_optimize: best energy, best bitstring
+// index, history length, converged flag per instance; params / best_params
+// hold every instance's parameters back to back in instance order.
+int rq_batch_results(rq_batch* h, double* best, uint64_t* best_index, uint32_t* iterations, int* converged,
+                     double* params, double* best_params) {
+  Batch* b = (Batch*)h;
+  if (!b || !b->ready || !b->o.hist) return fail(RQ_ERR_ARGUMENT, "batch has not been optimized");
+  const int ninst = (int)b->insts.size();
+  std::vector<int32_t> hl(ninst), cv(ninst);
+  std::vector<uint32_t> bi(ninst);
+  if (best) RQ_HIP(hipMemcpy(best, b->o.best, ninst * sizeof(double), hipMemcpyDeviceToHost));
+  if (best_index) {
+    RQ_HIP(hipMemcpy(bi.data(), b->o.best_idx, ninst * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (int k = 0; k < ninst; ++k) best_index[k] = bi[k];
+  }
+  if (iterations) {
+    RQ_HIP(hipMemcpy(hl.data(), b->o.hlen, ninst * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int k = 0; k < ninst; ++k) iterations[k] = (uint32_t)hl[k];
+  }
+  if (converged) {
+    RQ_HIP(hipMemcpy(cv.data(), b->o.conv, ninst * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int k = 0; k < ninst; ++k) converged[k] = cv[k];
+  }
+  if (params) RQ_HIP(hipMemcpy(params, b->d_params, b->params.size() * sizeof(double), hipMemcpyDeviceToHost));
+  if (best_params)
+    RQ_HIP(hipMemcpy(best_params, b->d_best_params, b->params.size() * sizeof(double), hipMemcpyDeviceToHost));
+  return RQ_OK;
+}
+
+// Exact maximum cut of every QAOA instance (brute force over all 2^n
+// bitstrings on the device, FP64 cut weights); 0 for VQE instances.
+int rq_batch_maxcut(rq_batch* h, double* out) {
+  Batch* b = (Batch*)h;
+  if (!b || !out) return fail(RQ_ERR_ARGUMENT, "null pointer");
+  int rc = finalize(*b);
+  if (rc) return rc;
+  const int ninst = (int)b->insts.size();
+  unsigned long long* d = nullptr;
+  RQ_HIP(hipMalloc((void**)&d, std::max(ninst, 1) * sizeof(unsigned long long)));
+  hipLaunchKernelGGL(k_zero_u32, dim3((2 * ninst + 255) / 256), dim3(256), 0, b->st, (unsigned*)d, 2 * ninst);
+  for (int k = 0; k < ninst; ++k) {
+    const BInst& in = b->insts[k];
+    if (in.kind != BK_QAOA) continue;
+    const unsigned grid = (unsigned)std::min<uint64_t>(1024, ((1ull << in.n) + 255) / 256);
+    hipLaunchKernelGGL(k_maxcut, dim3(grid), dim3(BATCH_THREADS), 0, b->st, in.n, b->d_edges + in.term_off,
+                       in.nterms, d + k);
+  }
+  RQ_HIP(hipGetLastError());
+  std::vector<unsigned long long> hv(ninst);
+  RQ_HIP(hipMemcpyAsync(hv.data(), d, ninst * sizeof(unsigned long long), hipMemcpyDeviceToHost, b->st));
+  RQ_HIP(hipStreamSynchronize(b->st));
+  (void)hipFree(d);
+  for (int k = 0; k < ninst; ++k) std::memcpy(&out[k], &hv[k], sizeof(double));  // non-negative: bits order = value order
   return RQ_OK;
 }
 

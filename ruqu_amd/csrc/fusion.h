@@ -78,6 +78,9 @@ struct FusionConfig {
 // exactly (via measure/reset on the state) between fused segments.
 int run_fused(State& s, const rq_gate* gates, size_t ngates, const double* draws, size_t ndraws, size_t* cursor,
               rq_measurement* meas, size_t cap, size_t* produced, FusionStats* st);
+// Same contract with dense 5-qubit blocks on the matrix cores (dense.cpp).
+int run_dense(State& s, const rq_gate* gates, size_t ngates, const double* draws, size_t ndraws, size_t* cursor,
+              rq_measurement* meas, size_t cap, size_t* produced, FusionStats* st);
 
 // Host-side plan of one fused segment (exposed for tests and the autotuner).
 struct FusedPlan {

@@ -578,6 +578,25 @@ __global__ __launch_bounds__(B) void k_cut_table(float* __restrict__ ctab, int n
     ctab[i] = (float)cut_of(i, e, ne);
 }
 
+// Maximum cut weight over all bitstrings; cut weights are non-negative, so
+// their IEEE bit patterns order like their values and the block maxima
+// combine with an integer atomic max (result independent of order).
+__global__ __launch_bounds__(B) void k_maxcut(int n, const BEdge* __restrict__ e, int ne,
+                                              unsigned long long* __restrict__ out) {
+  __shared__ double red[B];
+  const uint64_t N = 1ull << n;
+  double best = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * B + threadIdx.x; i < N; i += (uint64_t)gridDim.x * B)
+    best = fmax(best, cut_of(i, e, ne));
+  red[threadIdx.x] = best;
+  __syncthreads();
+  for (int s = B / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
+}
+
 __global__ void k_zero_u32(unsigned* __restrict__ p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;

@@ -1,0 +1,103 @@
+// Dense 5-qubit blocks on the matrix cores (complex-f32 states).
+//
+// A block is a 32x32 complex unitary U on qubits q0 < ... < q4.  Every group
+// of 32 amplitudes that differ only in those qubits is transformed y = U x.
+// With the real embedding W = [[Re U, -Im U], [Im U, Re U]] (64 x 64) the
+// groups form one real GEMM Y = X W^T (rows = groups, 64 columns = real and
+// imaginary parts), run on v_mfma_f32_32x32x2_f32: a wave takes 32 groups
+// (1024 amplitudes) at a time, 32 K-steps x 2 column halves = 64 matrix-core
+// instructions, 256 flop per amplitude, FP32 accumulation.
+//
+// Operand layout of v_mfma_f32_32x32x2_f32: lane l holds A[l % 32][l / 32]
+// and B[l / 32][l % 32]; its 16 results are column l % 32, rows
+// 8*(i/4) + 4*(l/32) + i%4.  Lane l of a wave therefore reads, for group row
+// l % 32, the 16 amplitudes whose block index c has c % 2 == l / 32 (their
+// real parts feed K-steps 0..15, their imaginary parts steps 16..31), and
+// writes output amplitude c = l % 32 of 16 groups.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "internal.h"
+
+namespace rq {
+namespace {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+struct BlockQ {
+  int q[5];
+};
+
+__device__ __forceinline__ uint64_t spread5(uint64_t g, const BlockQ& b) {
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    const int q = b.q[i];
+    g = ((g >> q) << (q + 1)) | (g & ((1ull << q) - 1ull));
+  }
+  return g;
+}
+
+__device__ __forceinline__ uint64_t offset5(int c, const BlockQ& b) {
+  uint64_t o = 0;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) o |= (uint64_t)((c >> i) & 1) << b.q[i];
+  return o;
+}
+
+// wt[(h*32 + s)*64 + l] = W[32h + l%32][2s + l/32] (the B operand of lane l
+// for K-step s and column half h), prepared on the host.
+__global__ __launch_bounds__(256) void k_dense5_f32(float2* __restrict__ a, uint64_t ntiles, BlockQ bq,
+                                                    const float* __restrict__ wt) {
+  const int lane = threadIdx.x & 63, row = lane & 31, half = lane >> 5;
+  const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint64_t nwaves = (uint64_t)gridDim.x * 4;
+  float b0[32], b1[32];
+#pragma unroll
+  for (int s = 0; s < 32; ++s) {
+    b0[s] = wt[s * 64 + lane];
+    b1[s] = wt[(32 + s) * 64 + lane];
+  }
+  uint64_t off[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) off[j] = offset5(2 * j + half, bq);
+  const uint64_t out_off = offset5(row, bq);
+  for (uint64_t tile = wave; tile < ntiles; tile += nwaves) {
+    const uint64_t base = spread5(tile * 32 + row, bq);
+    float2 x[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) x[j] = a[base + off[j]];
+    f32x16 acc0, acc1;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      acc0[i] = 0.0f;
+      acc1[i] = 0.0f;
+    }
+#pragma unroll
+    for (int s = 0; s < 32; ++s) {
+      const float av = s < 16 ? x[s].x : x[s - 16].y;
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, b0[s], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, b1[s], acc1, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int grow = 8 * (i >> 2) + 4 * half + (i & 3);
+      a[spread5(tile * 32 + grow, bq) + out_off] = make_float2(acc0[i], acc1[i]);
+    }
+  }
+}
+
+}  // namespace
+
+// Applies one dense 5-qubit block (qubits ascending, wt as above, resident on
+// the device) to a complex-f32 state of at least 10 qubits.
+void launch_dense5_f32(State& s, const int* q, const float* d_wt) {
+  BlockQ bq;
+  for (int i = 0; i < 5; ++i) bq.q[i] = q[i];
+  const uint64_t ntiles = (uint64_t)s.n >> 10;  // 32 groups of 32 amplitudes per wave-tile
+  uint64_t blocks = (ntiles + 3) / 4;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_dense5_f32, dim3((unsigned)blocks), dim3(256), 0, s.stream, (float2*)s.amps, ntiles, bq, d_wt);
+}
+
+}  // namespace rq

@@ -244,7 +244,8 @@ int run_gates(State& s, const rq_gate* gates, size_t ngates, const double* draws
   int rc = RQ_OK;
   if (fusion) {
     FusionStats st{};
-    rc = run_fused(s, gates, ngates, draws, ndraws, &cursor, meas, cap, &produced, &st);
+    rc = fusion == 2 ? run_dense(s, gates, ngates, draws, ndraws, &cursor, meas, cap, &produced, &st)
+                     : run_fused(s, gates, ngates, draws, ndraws, &cursor, meas, cap, &produced, &st);
     unitary = st.unitary;
     passes = st.passes;
     fused = st.fused;

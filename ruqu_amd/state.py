@@ -329,6 +329,14 @@ def _count_draws(arr):
     return int(np.count_nonzero((k == GateKind.Measure) | (k == GateKind.Reset)))
 
 
+def _fusion_mode(f):
+    """False/0: exact per-gate kernels; True/1/"fused": the fused tile
+    schedule; 2/"dense": dense 5-qubit blocks on the matrix cores."""
+    if isinstance(f, str):
+        return {"exact": 0, "fused": 1, "dense": 2}[f]
+    return int(f)
+
+
 class Simulator:
     """simulator.rs:47-164 (stateless)."""
 
@@ -357,7 +365,7 @@ class Simulator:
         nout = C.c_size_t(0)
         met = Metrics()
         check(lib().rq_run_circuit(state.handle, ptr(arr), len(arr), ptr(draws), len(draws), ptr(meas), len(meas),
-                                   C.byref(nout), 1 if config.fusion else 0, C.byref(met)))
+                                   C.byref(nout), _fusion_mode(config.fusion), C.byref(met)))
         outs = [MeasurementOutcome(r["qubit"], r["result"], r["probability"]) for r in meas[: nout.value]]
         return SimulationResult(state, outs, SimulationMetrics(met))
 

@@ -70,3 +70,13 @@ inline int min(int a, int b) { return a < b ? a : b; }
 inline int __popc(uint32_t x) { return __builtin_popcount(x); }
 inline int __popcll(uint64_t x) { return __builtin_popcountll(x); }
 inline unsigned atomicAdd(unsigned* p, unsigned v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned long long atomicMax(unsigned long long* p, unsigned long long v) {
+  const unsigned long long old = *p;
+  if (v > old) *p = v;
+  return old;
+}
+inline long long __double_as_longlong(double x) {
+  long long r;
+  memcpy(&r, &x, sizeof r);
+  return r;
+}
