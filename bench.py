@@ -146,6 +146,7 @@ def main():
     err = None
     if prec == "c32" and not args.no_f64_check and nq <= 31:
         log("f64 run of the same circuit for the f32 error ...")
+        step()  # the per-gate timing above changed the state: run the circuit again
         a32 = st.state_vector()
         st64 = QuantumState.new_with_seed(nq, args.seed, "c64")
         rc = L.rq_run_circuit(st64.handle, ptr(arr), ngates, None, 0, None, 0, C.byref(nout), 1, C.byref(met))
