@@ -66,6 +66,18 @@ def cpu_baseline(nq, arr, budget_s=20.0):
                       "(mixed_precision.rs:396-458), %.1f s" % (done, nq, el)}
 
 
+def measured_traffic(path):
+    """HBM bytes per launch of the dominant kernel, from the calibrated
+    FETCH_SIZE / WRITE_SIZE profile summary (None when absent)."""
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch"):
+                return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,7 +249,9 @@ def main():
                                (nq, "f32" if prec == "c32" else "f64", args.depth, ngates),
                    "qubits": nq, "depth": args.depth, "gates": ngates, "passes_per_step": passes},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": (measured_traffic(os.path.join(ROOT, "profiles", "fused_stage_30q_c32_hbm_traffic.md"))
+                                 if (nq, prec, args.depth) == (30, "c32", 40) else None),
                      "kernel": "k_fused_stage", "bytes_per_launch": stage_bytes,
                      "mean_launch_ms": round(mean_stage_ms, 4)},
         "per_gate_1q_GBps": per_gate,
