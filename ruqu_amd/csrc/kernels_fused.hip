@@ -23,7 +23,7 @@ namespace rq {
 namespace {
 
 template <class T>
-struct Cx {
+struct alignas(2 * sizeof(T)) Cx {  // one 8- / 16-byte LDS access per amplitude
   T re, im;
 };
 
@@ -48,8 +48,16 @@ struct Vec2<double> {
   using type = double2;
 };
 
-// LDS position of tile index t: one pad amplitude every 16 (bank spreading).
-__device__ __forceinline__ uint32_t lds_pos(uint32_t t) { return t + (t >> 4); }
+// LDS byte address of tile index t.  The XOR swizzle folds index bits 5..9
+// (f32; 4..7 for f64) into the bank bits, so the lanes of a wave hit distinct
+// banks whichever tile bits they hold.  It is linear over bits: the address of
+// (thread part | register part) is the XOR of the two parts' addresses, so a
+// phase computes its per-thread part once and each register adds one XOR.
+template <class T>
+__device__ __forceinline__ uint32_t lds_addr(uint32_t t) {
+  if constexpr (sizeof(T) == 4) return (t ^ ((t >> 5) & 31u)) * 8u;
+  else return (t ^ ((t >> 4) & 15u)) * 16u;
+}
 
 template <class T>
 __device__ __forceinline__ void load_amp(const T* __restrict__ a, uint64_t g, Cx<T>& v) {
@@ -67,15 +75,15 @@ __device__ __forceinline__ void store_amp(T* __restrict__ a, uint64_t g, const C
   reinterpret_cast<V*>(a)[g] = x;
 }
 
-// 2x2 on register bit B (controlled: only pairs whose k satisfies ctl_reg).
-template <int B, class T>
+// 2x2 on register bit B (CTL: only pairs whose k satisfies ctl_reg).
+template <int B, bool CTL, class T>
 __device__ __forceinline__ void u1(Cx<T> (&v)[FUSE_R], const Cx<T>* __restrict__ mp, uint32_t ctl_reg) {
   const Cx<T> m00 = mp[0], m01 = mp[1], m10 = mp[2], m11 = mp[3];
 #pragma unroll
   for (int k = 0; k < FUSE_R; ++k) {
     if (k & (1 << B)) continue;
     const int k1 = k | (1 << B);
-    if ((k & ctl_reg) != ctl_reg) continue;
+    if (CTL && (k & ctl_reg) != ctl_reg) continue;
     Cx<T> x = v[k], y = v[k1];
     v[k] = cmadd(m00, x, m01, y);
     v[k1] = cmadd(m10, x, m11, y);
@@ -137,13 +145,13 @@ __device__ __forceinline__ void dispatch_cx(Cx<T> (&v)[FUSE_R], int b, uint32_t 
   }
 }
 
-template <class T>
+template <bool CTL, class T>
 __device__ __forceinline__ void dispatch_u1(Cx<T> (&v)[FUSE_R], int b, const Cx<T>* mp, uint32_t ctl_reg) {
   switch (b) {
-    case 0: u1<0>(v, mp, ctl_reg); break;
-    case 1: u1<1>(v, mp, ctl_reg); break;
-    case 2: u1<2>(v, mp, ctl_reg); break;
-    default: u1<3>(v, mp, ctl_reg); break;
+    case 0: u1<0, CTL>(v, mp, ctl_reg); break;
+    case 1: u1<1, CTL>(v, mp, ctl_reg); break;
+    case 2: u1<2, CTL>(v, mp, ctl_reg); break;
+    default: u1<3, CTL>(v, mp, ctl_reg); break;
   }
 }
 
@@ -165,11 +173,83 @@ __device__ __forceinline__ void dispatch_u2(Cx<T> (&v)[FUSE_R], int b0, int b1, 
   }
 }
 
-// Bit of the amplitude index at a phase-local location.
-__device__ __forceinline__ int loc_bit(int loc, int pos, int k, uint32_t tid, uint64_t outer) {
-  if (loc == LOC_REG) return (k >> pos) & 1;
-  if (loc == LOC_THR) return (int)((tid >> pos) & 1u);
-  return (int)((outer >> pos) & 1ull);
+// Register-bit pattern: bit k of the result is bit `pos` of k.
+__device__ __forceinline__ uint32_t reg_pattern(int pos) {
+  return (uint32_t)((0xFF00F0F0CCCCAAAAull >> (16 * pos)) & 0xFFFFull);
+}
+
+// A diagonal index bit of register k: ((kmask >> k) & 1) when the bit is a
+// register bit, else the per-thread / per-tile constant cbit.
+struct DBit {
+  uint32_t kmask;  // register pattern (0 unless the bit is a register bit)
+  uint32_t cbit;   // thread or outer bit value (0 for register bits)
+  bool reg;
+};
+__device__ __forceinline__ DBit dbit(int loc, int pos, uint32_t tid, uint64_t outer) {
+  DBit d;
+  d.reg = loc == LOC_REG;
+  d.kmask = d.reg ? reg_pattern(pos) : 0u;
+  d.cbit = loc == LOC_THR ? ((tid >> pos) & 1u) : (loc == LOC_OUT ? (uint32_t)((outer >> pos) & 1ull) : 0u);
+  return d;
+}
+
+template <class T>
+__device__ __forceinline__ Cx<T> neg_if(Cx<T> a, bool c) {
+  return c ? Cx<T>{-a.re, -a.im} : a;
+}
+
+template <class T>
+__device__ __forceinline__ void apply_cz(Cx<T> (&v)[FUSE_R], const DBit& b0, const DBit& b1) {
+  if (b0.reg && b1.reg) {
+    const uint32_t m = b0.kmask & b1.kmask;  // uniform
+#pragma unroll
+    for (int k = 0; k < FUSE_R; ++k)
+      if ((m >> k) & 1u) v[k] = Cx<T>{-v[k].re, -v[k].im};
+  } else if (b0.reg || b1.reg) {
+    const uint32_t m = b0.kmask | b1.kmask;
+    const bool c = (b0.cbit | b1.cbit) != 0u;  // the non-register bit (per thread / tile)
+#pragma unroll
+    for (int k = 0; k < FUSE_R; ++k)
+      if ((m >> k) & 1u) v[k] = neg_if(v[k], c);
+  } else {
+    const bool c = (b0.cbit & b1.cbit) != 0u;
+#pragma unroll
+    for (int k = 0; k < FUSE_R; ++k) v[k] = neg_if(v[k], c);
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void apply_d1(Cx<T> (&v)[FUSE_R], const Cx<T>* __restrict__ mp, const DBit& b0) {
+  if (b0.reg) {
+    const Cx<T> f0 = mp[0], f1 = mp[1];
+#pragma unroll
+    for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(((b0.kmask >> k) & 1u) ? f1 : f0, v[k]);
+  } else {
+    const Cx<T> f = b0.cbit ? mp[1] : mp[0];
+#pragma unroll
+    for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(f, v[k]);
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void apply_d2(Cx<T> (&v)[FUSE_R], const Cx<T>* __restrict__ mp, const DBit& b0,
+                                         const DBit& b1) {
+  if (b0.reg && b1.reg) {
+#pragma unroll
+    for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(mp[((b0.kmask >> k) & 1u) * 2 + ((b1.kmask >> k) & 1u)], v[k]);
+  } else if (b0.reg) {
+    const Cx<T> f0 = mp[b1.cbit], f1 = mp[2 + b1.cbit];
+#pragma unroll
+    for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(((b0.kmask >> k) & 1u) ? f1 : f0, v[k]);
+  } else if (b1.reg) {
+    const Cx<T> f0 = mp[b0.cbit * 2], f1 = mp[b0.cbit * 2 + 1];
+#pragma unroll
+    for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(((b1.kmask >> k) & 1u) ? f1 : f0, v[k]);
+  } else {
+    const Cx<T> f = mp[b0.cbit * 2 + b1.cbit];
+#pragma unroll
+    for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(f, v[k]);
+  }
 }
 
 template <class T>
@@ -180,8 +260,11 @@ __device__ __forceinline__ void apply_ops(Cx<T> (&v)[FUSE_R], const FOp* __restr
     const Cx<T>* mp = pool + op.mat;
     switch (op.kind) {
       case FOP_U1:
-        if ((tid & op.ctl_thr) == op.ctl_thr && (outer & op.ctl_out) == op.ctl_out)
-          dispatch_u1(v, op.t0, mp, op.ctl_reg);
+        if (op.ctl_reg == 0u && op.ctl_thr == 0u && op.ctl_out == 0ull) {
+          dispatch_u1<false>(v, op.t0, mp, 0u);
+        } else if ((tid & op.ctl_thr) == op.ctl_thr && (outer & op.ctl_out) == op.ctl_out) {
+          dispatch_u1<true>(v, op.t0, mp, op.ctl_reg);
+        }
         break;
       case FOP_U2:
         dispatch_u2(v, op.t0, op.t1, mp);
@@ -190,21 +273,13 @@ __device__ __forceinline__ void apply_ops(Cx<T> (&v)[FUSE_R], const FOp* __restr
         if ((tid & op.ctl_thr) == op.ctl_thr && (outer & op.ctl_out) == op.ctl_out) dispatch_cx(v, op.t0, op.ctl_reg);
         break;
       case FOP_CZ:
-#pragma unroll
-        for (int k = 0; k < FUSE_R; ++k)
-          if (loc_bit(op.dloc0, op.dpos0, k, tid, outer) & loc_bit(op.dloc1, op.dpos1, k, tid, outer))
-            v[k] = Cx<T>{-v[k].re, -v[k].im};
+        apply_cz(v, dbit(op.dloc0, op.dpos0, tid, outer), dbit(op.dloc1, op.dpos1, tid, outer));
         break;
       case FOP_D1:
-#pragma unroll
-        for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(mp[loc_bit(op.dloc0, op.dpos0, k, tid, outer)], v[k]);
+        apply_d1(v, mp, dbit(op.dloc0, op.dpos0, tid, outer));
         break;
       default:  // FOP_D2
-#pragma unroll
-        for (int k = 0; k < FUSE_R; ++k) {
-          int i = loc_bit(op.dloc0, op.dpos0, k, tid, outer) * 2 + loc_bit(op.dloc1, op.dpos1, k, tid, outer);
-          v[k] = cmul(mp[i], v[k]);
-        }
+        apply_d2(v, mp, dbit(op.dloc0, op.dpos0, tid, outer), dbit(op.dloc1, op.dpos1, tid, outer));
         break;
     }
   }
@@ -218,6 +293,15 @@ struct Map {
   uint32_t treg[FUSE_R_BITS];
   uint64_t greg[FUSE_R_BITS];
 };
+
+template <class T>
+__device__ __forceinline__ Cx<T> lds_ld(const Cx<T>* lds, uint32_t byte) {
+  return *reinterpret_cast<const Cx<T>*>(reinterpret_cast<const unsigned char*>(lds) + byte);
+}
+template <class T>
+__device__ __forceinline__ void lds_st(Cx<T>* lds, uint32_t byte, const Cx<T>& x) {
+  *reinterpret_cast<Cx<T>*>(reinterpret_cast<unsigned char*>(lds) + byte) = x;
+}
 
 __device__ __forceinline__ Map make_map(const FStage& st, const int* regt, const int* thrt, uint32_t tid,
                                         uint64_t outer) {
@@ -275,21 +359,23 @@ struct Unroll {
     Unroll<K - 1, T>::scatter_g(v, a, mp);
     store_amp(a, mp.gthr | greg_of<K - 1>(mp), v[K - 1]);
   }
-  static __device__ __forceinline__ void gather_l(Cx<T> (&v)[FUSE_R], const Cx<T>* lds, const Map& mp) {
-    Unroll<K - 1, T>::gather_l(v, lds, mp);
-    v[K - 1] = lds[lds_pos(mp.tthr | treg_of<K - 1>(mp))];
+  static __device__ __forceinline__ void gather_l(Cx<T> (&v)[FUSE_R], const Cx<T>* lds, const Map& mp,
+                                                  uint32_t lt) {
+    Unroll<K - 1, T>::gather_l(v, lds, mp, lt);
+    v[K - 1] = lds_ld(lds, lt ^ lds_addr<T>(treg_of<K - 1>(mp)));
   }
-  static __device__ __forceinline__ void scatter_l(const Cx<T> (&v)[FUSE_R], Cx<T>* lds, const Map& mp) {
-    Unroll<K - 1, T>::scatter_l(v, lds, mp);
-    lds[lds_pos(mp.tthr | treg_of<K - 1>(mp))] = v[K - 1];
+  static __device__ __forceinline__ void scatter_l(const Cx<T> (&v)[FUSE_R], Cx<T>* lds, const Map& mp,
+                                                   uint32_t lt) {
+    Unroll<K - 1, T>::scatter_l(v, lds, mp, lt);
+    lds_st(lds, lt ^ lds_addr<T>(treg_of<K - 1>(mp)), v[K - 1]);
   }
 };
 template <class T>
 struct Unroll<0, T> {
   static __device__ __forceinline__ void gather_g(Cx<T> (&)[FUSE_R], const T*, const Map&) {}
   static __device__ __forceinline__ void scatter_g(const Cx<T> (&)[FUSE_R], T*, const Map&) {}
-  static __device__ __forceinline__ void gather_l(Cx<T> (&)[FUSE_R], const Cx<T>*, const Map&) {}
-  static __device__ __forceinline__ void scatter_l(const Cx<T> (&)[FUSE_R], Cx<T>*, const Map&) {}
+  static __device__ __forceinline__ void gather_l(Cx<T> (&)[FUSE_R], const Cx<T>*, const Map&, uint32_t) {}
+  static __device__ __forceinline__ void scatter_l(const Cx<T> (&)[FUSE_R], Cx<T>*, const Map&, uint32_t) {}
 };
 
 template <class T>
@@ -316,13 +402,13 @@ __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FS
       } else {
         const Map nat = make_natural_map(st, tid, outer);
         Unroll<FUSE_R, T>::gather_g(v, a, nat);
-        Unroll<FUSE_R, T>::scatter_l(v, lds, nat);
+        Unroll<FUSE_R, T>::scatter_l(v, lds, nat, lds_addr<T>(nat.tthr));
         __syncthreads();
-        Unroll<FUSE_R, T>::gather_l(v, lds, mp);
+        Unroll<FUSE_R, T>::gather_l(v, lds, mp, lds_addr<T>(mp.tthr));
         __syncthreads();
       }
     } else {
-      Unroll<FUSE_R, T>::gather_l(v, lds, mp);
+      Unroll<FUSE_R, T>::gather_l(v, lds, mp, lds_addr<T>(mp.tthr));
       __syncthreads();
     }
     apply_ops(v, ops, ph.op_begin, ph.op_end, pool, tid, outer);
@@ -331,13 +417,13 @@ __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FS
         Unroll<FUSE_R, T>::scatter_g(v, a, mp);
       } else {
         const Map nat = make_natural_map(st, tid, outer);
-        Unroll<FUSE_R, T>::scatter_l(v, lds, mp);
+        Unroll<FUSE_R, T>::scatter_l(v, lds, mp, lds_addr<T>(mp.tthr));
         __syncthreads();
-        Unroll<FUSE_R, T>::gather_l(v, lds, nat);
+        Unroll<FUSE_R, T>::gather_l(v, lds, nat, lds_addr<T>(nat.tthr));
         Unroll<FUSE_R, T>::scatter_g(v, a, nat);
       }
     } else {
-      Unroll<FUSE_R, T>::scatter_l(v, lds, mp);
+      Unroll<FUSE_R, T>::scatter_l(v, lds, mp, lds_addr<T>(mp.tthr));
       __syncthreads();
     }
   }
@@ -380,7 +466,7 @@ int launch_plan_t(State& s, const FusedPlan& plan) {
     const FStage& st = plan.stages[i];
     const unsigned threads = 1u << (st.m - FUSE_R_BITS);
     const uint64_t tiles = 1ull << st.nouter;
-    const size_t lds = ((size_t(1) << st.m) + ((size_t(1) << st.m) >> 4)) * sizeof(Cx<T>);
+    const size_t lds = (size_t(1) << st.m) * sizeof(Cx<T>);
     hipLaunchKernelGGL(k_fused_stage<T>, dim3((unsigned)tiles), dim3(threads), lds, s.stream, (T*)s.amps,
                        reinterpret_cast<const FStage*>(base + o_st + i * sizeof(FStage)),
                        reinterpret_cast<const FPhase*>(base + o_ph), reinterpret_cast<const FOp*>(base + o_op),
@@ -396,7 +482,7 @@ int launch_fused_plan(State& s, const FusedPlan& plan) {
   if (plan.stages.empty()) return RQ_OK;
   for (const FStage& st : plan.stages)
     if (st.m < FUSE_R_BITS || st.m > FUSE_MAX_M || st.nouter > 31 ||
-        ((size_t(1) << st.m) + ((size_t(1) << st.m) >> 4)) * s.amp_bytes() > 160 * 1024)
+        (size_t(1) << st.m) * s.amp_bytes() > 160 * 1024)
       return fail(RQ_ERR_ARGUMENT, "fused stage shape out of range");
   if (s.prec == RQ_C64) return launch_plan_t<double>(s, plan);
   return launch_plan_t<float>(s, plan);
