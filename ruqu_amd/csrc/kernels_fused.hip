@@ -173,24 +173,19 @@ __device__ __forceinline__ void dispatch_u2(Cx<T> (&v)[FUSE_R], int b0, int b1, 
   }
 }
 
-// Register-bit pattern: bit k of the result is bit `pos` of k.
-__device__ __forceinline__ uint32_t reg_pattern(int pos) {
-  return (uint32_t)((0xFF00F0F0CCCCAAAAull >> (16 * pos)) & 0xFFFFull);
-}
-
-// A diagonal index bit of register k: ((kmask >> k) & 1) when the bit is a
-// register bit, else the per-thread / per-tile constant cbit.
-struct DBit {
-  uint32_t kmask;  // register pattern (0 unless the bit is a register bit)
-  uint32_t cbit;   // thread or outer bit value (0 for register bits)
-  bool reg;
+// Packed op record (built from FOp at launch, one 8-byte scalar load per op):
+//   x: kind:3 | t0:2 | t1:2 | cloc:2 | cpos:5 | dloc0:2 | dpos0:6 | dloc1:2 | dpos1:6
+//   y: matrix / diagonal offset in the pool
+// Locations: 0 none, LOC_REG + 1 register bit, LOC_THR + 1 thread bit,
+// LOC_OUT + 1 tile-id bit (position = bit of blockIdx.x).
+struct POp {
+  uint32_t x, y;
 };
-__device__ __forceinline__ DBit dbit(int loc, int pos, uint32_t tid, uint64_t outer) {
-  DBit d;
-  d.reg = loc == LOC_REG;
-  d.kmask = d.reg ? reg_pattern(pos) : 0u;
-  d.cbit = loc == LOC_THR ? ((tid >> pos) & 1u) : (loc == LOC_OUT ? (uint32_t)((outer >> pos) & 1ull) : 0u);
-  return d;
+constexpr int PL_NONE = 0, PL_REG = LOC_REG + 1, PL_THR = LOC_THR + 1, PL_OUT = LOC_OUT + 1;
+
+// Value of a non-register location bit for this thread / tile.
+__device__ __forceinline__ uint32_t loc_val(int loc, int pos, uint32_t tid) {
+  return loc == PL_THR ? ((tid >> pos) & 1u) : ((blockIdx.x >> pos) & 1u);
 }
 
 template <class T>
@@ -198,88 +193,150 @@ __device__ __forceinline__ Cx<T> neg_if(Cx<T> a, bool c) {
   return c ? Cx<T>{-a.re, -a.im} : a;
 }
 
+// Controlled swap on register bit B with register control bit C.
+template <int B, int C, class T>
+__device__ __forceinline__ void cx_rc(Cx<T> (&v)[FUSE_R]) {
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) {
+    if ((k & (1 << B)) || !(k & (1 << C))) continue;
+    const Cx<T> x = v[k];
+    v[k] = v[k | (1 << B)];
+    v[k | (1 << B)] = x;
+  }
+}
+template <int B, class T>
+__device__ __forceinline__ void cx_all(Cx<T> (&v)[FUSE_R]) {
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) {
+    if (k & (1 << B)) continue;
+    const Cx<T> x = v[k];
+    v[k] = v[k | (1 << B)];
+    v[k | (1 << B)] = x;
+  }
+}
+
+// CZ with both bits in registers: negate where bits P0 and P1 of k are set.
+template <int P0, int P1, class T>
+__device__ __forceinline__ void cz_rr(Cx<T> (&v)[FUSE_R]) {
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k)
+    if (((k >> P0) & (k >> P1)) & 1) v[k] = Cx<T>{-v[k].re, -v[k].im};
+}
+// CZ with one register bit P: negate where bit P of k is set and c holds.
+template <int P, class T>
+__device__ __forceinline__ void cz_r(Cx<T> (&v)[FUSE_R], bool c) {
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k)
+    if ((k >> P) & 1) v[k] = neg_if(v[k], c);
+}
+// Diagonal 2-entry factor selected by register bit P.
+template <int P, class T>
+__device__ __forceinline__ void dsel_r(Cx<T> (&v)[FUSE_R], Cx<T> f0, Cx<T> f1) {
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(((k >> P) & 1) ? f1 : f0, v[k]);
+}
+// Diagonal 4-entry factor indexed by register bits (P0, P1).
+template <int P0, int P1, class T>
+__device__ __forceinline__ void d2_rr(Cx<T> (&v)[FUSE_R], const Cx<T>* __restrict__ mp) {
+  const Cx<T> f0 = mp[0], f1 = mp[1], f2 = mp[2], f3 = mp[3];
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) {
+    const int i = ((k >> P0) & 1) * 2 + ((k >> P1) & 1);
+    v[k] = cmul(i == 0 ? f0 : (i == 1 ? f1 : (i == 2 ? f2 : f3)), v[k]);
+  }
+}
+
+#define RQ_P2(F, ...)                      \
+  switch (p0 * 4 + p1) {                   \
+    case 1: F<0, 1>(__VA_ARGS__); break;   \
+    case 2: F<0, 2>(__VA_ARGS__); break;   \
+    case 3: F<0, 3>(__VA_ARGS__); break;   \
+    case 4: F<1, 0>(__VA_ARGS__); break;   \
+    case 6: F<1, 2>(__VA_ARGS__); break;   \
+    case 7: F<1, 3>(__VA_ARGS__); break;   \
+    case 8: F<2, 0>(__VA_ARGS__); break;   \
+    case 9: F<2, 1>(__VA_ARGS__); break;   \
+    case 11: F<2, 3>(__VA_ARGS__); break;  \
+    case 12: F<3, 0>(__VA_ARGS__); break;  \
+    case 13: F<3, 1>(__VA_ARGS__); break;  \
+    default: F<3, 2>(__VA_ARGS__); break;  \
+  }
+#define RQ_P1(F, P, ...)                 \
+  switch (P) {                           \
+    case 0: F<0>(__VA_ARGS__); break;    \
+    case 1: F<1>(__VA_ARGS__); break;    \
+    case 2: F<2>(__VA_ARGS__); break;    \
+    default: F<3>(__VA_ARGS__); break;   \
+  }
+
 template <class T>
-__device__ __forceinline__ void apply_cz(Cx<T> (&v)[FUSE_R], const DBit& b0, const DBit& b1) {
-  if (b0.reg && b1.reg) {
-    const uint32_t m = b0.kmask & b1.kmask;  // uniform
-#pragma unroll
-    for (int k = 0; k < FUSE_R; ++k)
-      if ((m >> k) & 1u) v[k] = Cx<T>{-v[k].re, -v[k].im};
-  } else if (b0.reg || b1.reg) {
-    const uint32_t m = b0.kmask | b1.kmask;
-    const bool c = (b0.cbit | b1.cbit) != 0u;  // the non-register bit (per thread / tile)
-#pragma unroll
-    for (int k = 0; k < FUSE_R; ++k)
-      if ((m >> k) & 1u) v[k] = neg_if(v[k], c);
-  } else {
-    const bool c = (b0.cbit & b1.cbit) != 0u;
-#pragma unroll
-    for (int k = 0; k < FUSE_R; ++k) v[k] = neg_if(v[k], c);
+__device__ __forceinline__ void apply_cx(Cx<T> (&v)[FUSE_R], int b, int cl, int cp, uint32_t tid) {
+  if (cl == PL_REG) {
+    const int p0 = b, p1 = cp;
+    RQ_P2(cx_rc, v)
+  } else if (cl == PL_NONE || loc_val(cl, cp, tid)) {
+    RQ_P1(cx_all, b, v)
   }
 }
 
 template <class T>
-__device__ __forceinline__ void apply_d1(Cx<T> (&v)[FUSE_R], const Cx<T>* __restrict__ mp, const DBit& b0) {
-  if (b0.reg) {
-    const Cx<T> f0 = mp[0], f1 = mp[1];
-#pragma unroll
-    for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(((b0.kmask >> k) & 1u) ? f1 : f0, v[k]);
-  } else {
-    const Cx<T> f = b0.cbit ? mp[1] : mp[0];
-#pragma unroll
-    for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(f, v[k]);
-  }
-}
-
-template <class T>
-__device__ __forceinline__ void apply_d2(Cx<T> (&v)[FUSE_R], const Cx<T>* __restrict__ mp, const DBit& b0,
-                                         const DBit& b1) {
-  if (b0.reg && b1.reg) {
-#pragma unroll
-    for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(mp[((b0.kmask >> k) & 1u) * 2 + ((b1.kmask >> k) & 1u)], v[k]);
-  } else if (b0.reg) {
-    const Cx<T> f0 = mp[b1.cbit], f1 = mp[2 + b1.cbit];
-#pragma unroll
-    for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(((b0.kmask >> k) & 1u) ? f1 : f0, v[k]);
-  } else if (b1.reg) {
-    const Cx<T> f0 = mp[b0.cbit * 2], f1 = mp[b0.cbit * 2 + 1];
-#pragma unroll
-    for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(((b1.kmask >> k) & 1u) ? f1 : f0, v[k]);
-  } else {
-    const Cx<T> f = mp[b0.cbit * 2 + b1.cbit];
-#pragma unroll
-    for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(f, v[k]);
-  }
-}
-
-template <class T>
-__device__ __forceinline__ void apply_ops(Cx<T> (&v)[FUSE_R], const FOp* __restrict__ ops, int b, int e,
-                                          const Cx<T>* __restrict__ pool, uint32_t tid, uint64_t outer) {
+__device__ __forceinline__ void apply_ops(Cx<T> (&v)[FUSE_R], const POp* __restrict__ ops, int b, int e,
+                                          const Cx<T>* __restrict__ pool, uint32_t tid) {
   for (int o = b; o < e; ++o) {
-    const FOp& op = ops[o];
-    const Cx<T>* mp = pool + op.mat;
-    switch (op.kind) {
+    const POp w = ops[o];
+    const int kind = w.x & 7u, t0 = (w.x >> 3) & 3u, t1 = (w.x >> 5) & 3u, cl = (w.x >> 7) & 3u,
+              cp = (w.x >> 9) & 31u, dl0 = (w.x >> 14) & 3u, dp0 = (w.x >> 16) & 63u, dl1 = (w.x >> 22) & 3u,
+              dp1 = (w.x >> 24) & 63u;
+    const Cx<T>* mp = pool + w.y;
+    switch (kind) {
       case FOP_U1:
-        if (op.ctl_reg == 0u && op.ctl_thr == 0u && op.ctl_out == 0ull) {
-          dispatch_u1<false>(v, op.t0, mp, 0u);
-        } else if ((tid & op.ctl_thr) == op.ctl_thr && (outer & op.ctl_out) == op.ctl_out) {
-          dispatch_u1<true>(v, op.t0, mp, op.ctl_reg);
-        }
+        if (cl == PL_NONE) dispatch_u1<false>(v, t0, mp, 0u);
+        else if (cl == PL_REG) dispatch_u1<true>(v, t0, mp, 1u << cp);
+        else if (loc_val(cl, cp, tid)) dispatch_u1<false>(v, t0, mp, 0u);
         break;
       case FOP_U2:
-        dispatch_u2(v, op.t0, op.t1, mp);
+        dispatch_u2(v, t0, t1, mp);
         break;
       case FOP_CX:
-        if ((tid & op.ctl_thr) == op.ctl_thr && (outer & op.ctl_out) == op.ctl_out) dispatch_cx(v, op.t0, op.ctl_reg);
+        apply_cx(v, t0, cl, cp, tid);
         break;
       case FOP_CZ:
-        apply_cz(v, dbit(op.dloc0, op.dpos0, tid, outer), dbit(op.dloc1, op.dpos1, tid, outer));
+        if (dl0 == PL_REG && dl1 == PL_REG) {
+          const int p0 = dp0, p1 = dp1;
+          RQ_P2(cz_rr, v)
+        } else if (dl0 == PL_REG) {
+          RQ_P1(cz_r, dp0, v, loc_val(dl1, dp1, tid) != 0u)
+        } else if (dl1 == PL_REG) {
+          RQ_P1(cz_r, dp1, v, loc_val(dl0, dp0, tid) != 0u)
+        } else if (loc_val(dl0, dp0, tid) & loc_val(dl1, dp1, tid)) {
+#pragma unroll
+          for (int k = 0; k < FUSE_R; ++k) v[k] = Cx<T>{-v[k].re, -v[k].im};
+        }
         break;
       case FOP_D1:
-        apply_d1(v, mp, dbit(op.dloc0, op.dpos0, tid, outer));
+        if (dl0 == PL_REG) {
+          RQ_P1(dsel_r, dp0, v, mp[0], mp[1])
+        } else {
+          const Cx<T> f = mp[loc_val(dl0, dp0, tid)];
+#pragma unroll
+          for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(f, v[k]);
+        }
         break;
       default:  // FOP_D2
-        apply_d2(v, mp, dbit(op.dloc0, op.dpos0, tid, outer), dbit(op.dloc1, op.dpos1, tid, outer));
+        if (dl0 == PL_REG && dl1 == PL_REG) {
+          const int p0 = dp0, p1 = dp1;
+          RQ_P2(d2_rr, v, mp)
+        } else if (dl0 == PL_REG) {
+          const uint32_t c1 = loc_val(dl1, dp1, tid);
+          RQ_P1(dsel_r, dp0, v, mp[c1], mp[2 + c1])
+        } else if (dl1 == PL_REG) {
+          const uint32_t c0 = loc_val(dl0, dp0, tid);
+          RQ_P1(dsel_r, dp1, v, mp[2 * c0], mp[2 * c0 + 1])
+        } else {
+          const Cx<T> f = mp[2 * loc_val(dl0, dp0, tid) + loc_val(dl1, dp1, tid)];
+#pragma unroll
+          for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(f, v[k]);
+        }
         break;
     }
   }
@@ -380,7 +437,7 @@ struct Unroll<0, T> {
 
 template <class T>
 __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FStage* __restrict__ stp,
-                                                      const FPhase* __restrict__ phases, const FOp* __restrict__ ops,
+                                                      const FPhase* __restrict__ phases, const POp* __restrict__ ops,
                                                       const Cx<T>* __restrict__ pool) {
   extern __shared__ unsigned char smem_raw[];
   Cx<T>* lds = reinterpret_cast<Cx<T>*>(smem_raw);
@@ -411,7 +468,7 @@ __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FS
       Unroll<FUSE_R, T>::gather_l(v, lds, mp, lds_addr<T>(mp.tthr));
       __syncthreads();
     }
-    apply_ops(v, ops, ph.op_begin, ph.op_end, pool, tid, outer);
+    apply_ops(v, ops, ph.op_begin, ph.op_end, pool, tid);
     if (p == np - 1) {
       if (ph.direct) {
         Unroll<FUSE_R, T>::scatter_g(v, a, mp);
@@ -429,12 +486,66 @@ __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FS
   }
 }
 
+// Packs the plan's ops for the kernel: one 8-byte record each, control and
+// diagonal positions of outer qubits turned into tile-id (blockIdx) bits.
+int pack_ops(const FusedPlan& plan, std::vector<POp>& out) {
+  out.assign(plan.ops.size(), POp{0u, 0u});
+  for (const FStage& st : plan.stages) {
+    auto tile_bit = [&](int q) {
+      for (int j = 0; j < st.nouter; ++j)
+        if (st.outer[j] == q) return j;
+      return -1;
+    };
+    auto enc = [&](int loc, int pos, uint32_t* l, uint32_t* p) {
+      if (loc == LOC_OUT) {
+        const int j = tile_bit(pos);
+        if (j < 0) return false;
+        pos = j;
+      }
+      *l = (uint32_t)loc + 1u;
+      *p = (uint32_t)pos;
+      return pos >= 0 && pos < 32;
+    };
+    for (int ph = st.phase0; ph < st.phase0 + st.nphase; ++ph) {
+      for (int o = plan.phases[ph].op_begin; o < plan.phases[ph].op_end; ++o) {
+        const FOp& op = plan.ops[o];
+        uint32_t cl = 0, cp = 0, dl0 = 0, dp0 = 0, dl1 = 0, dp1 = 0;
+        if (op.kind == FOP_U1 || op.kind == FOP_CX) {
+          const int nctl = __builtin_popcount(op.ctl_reg) + __builtin_popcount(op.ctl_thr) +
+                           __builtin_popcountll(op.ctl_out);
+          if (nctl > 1) return fail(RQ_ERR_ARGUMENT, "fused op with more than one control");
+          if (op.ctl_reg) { cl = PL_REG; cp = (uint32_t)__builtin_ctz(op.ctl_reg); }
+          else if (op.ctl_thr) { cl = PL_THR; cp = (uint32_t)__builtin_ctz(op.ctl_thr); }
+          else if (op.ctl_out) {
+            const int j = tile_bit(__builtin_ctzll(op.ctl_out));
+            if (j < 0) return fail(RQ_ERR_ARGUMENT, "fused control outside the tile id");
+            cl = PL_OUT;
+            cp = (uint32_t)j;
+          }
+        }
+        if (op.kind == FOP_D1 || op.kind == FOP_D2 || op.kind == FOP_CZ) {
+          if (!enc(op.dloc0, op.dpos0, &dl0, &dp0)) return fail(RQ_ERR_ARGUMENT, "fused diagonal bit not located");
+          if (op.kind != FOP_D1 && !enc(op.dloc1, op.dpos1, &dl1, &dp1))
+            return fail(RQ_ERR_ARGUMENT, "fused diagonal bit not located");
+        }
+        POp& w = out[o];
+        w.x = (uint32_t)op.kind | ((uint32_t)op.t0 << 3) | ((uint32_t)op.t1 << 5) | (cl << 7) | (cp << 9) |
+              (dl0 << 14) | (dp0 << 16) | (dl1 << 22) | (dp1 << 24);
+        w.y = (uint32_t)op.mat;
+      }
+    }
+  }
+  return RQ_OK;
+}
+
 template <class T>
 int launch_plan_t(State& s, const FusedPlan& plan) {
+  std::vector<POp> pops;
+  if (int rc = pack_ops(plan, pops)) return rc;
   const size_t nst = plan.stages.size(), nph = plan.phases.size(), nop = plan.ops.size(), npool = plan.pool.size();
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t o_st = 0, o_ph = al(nst * sizeof(FStage)), o_op = o_ph + al(nph * sizeof(FPhase)),
-               o_pool = o_op + al(nop * sizeof(FOp)), total = o_pool + al(npool * sizeof(Cx<T>));
+               o_pool = o_op + al(nop * sizeof(POp)), total = o_pool + al(npool * sizeof(Cx<T>));
   if (total > s.prog_cap) {
     if (s.prog) {
       RQ_HIP(hipStreamSynchronize(s.stream));
@@ -449,7 +560,7 @@ int launch_plan_t(State& s, const FusedPlan& plan) {
   std::vector<unsigned char> host(total, 0);
   std::memcpy(host.data() + o_st, plan.stages.data(), nst * sizeof(FStage));
   std::memcpy(host.data() + o_ph, plan.phases.data(), nph * sizeof(FPhase));
-  std::memcpy(host.data() + o_op, plan.ops.data(), nop * sizeof(FOp));
+  std::memcpy(host.data() + o_op, pops.data(), nop * sizeof(POp));
   Cx<T>* pp = reinterpret_cast<Cx<T>*>(host.data() + o_pool);
   for (size_t k = 0; k < npool; ++k) pp[k] = Cx<T>{(T)plan.pool[k].re, (T)plan.pool[k].im};
   // The host buffer is pageable: wait for the copy before it goes out of scope.
@@ -469,7 +580,7 @@ int launch_plan_t(State& s, const FusedPlan& plan) {
     const size_t lds = (size_t(1) << st.m) * sizeof(Cx<T>);
     hipLaunchKernelGGL(k_fused_stage<T>, dim3((unsigned)tiles), dim3(threads), lds, s.stream, (T*)s.amps,
                        reinterpret_cast<const FStage*>(base + o_st + i * sizeof(FStage)),
-                       reinterpret_cast<const FPhase*>(base + o_ph), reinterpret_cast<const FOp*>(base + o_op),
+                       reinterpret_cast<const FPhase*>(base + o_ph), reinterpret_cast<const POp*>(base + o_op),
                        reinterpret_cast<const Cx<T>*>(base + o_pool));
   }
   RQ_HIP(hipGetLastError());
