@@ -37,9 +37,12 @@ def sources():
 # Per-file extra flags.  The fused kernel dispatches on runtime register bits
 # through switches whose bodies index a register array with constants;
 # SimplifyCFG's common-instruction sinking would merge those bodies into
-# pointer phis and force the array into scratch memory.
+# pointer phis and force the array into scratch memory.  Its op dispatch is
+# wave-uniform (scalar op records): left unstructurized, the switches compile
+# to plain scalar branch trees instead of predicate-flag chains.
 EXTRA = {
-    "kernels_fused.hip": ["-mllvm", "-sink-common-insts=false"],
+    "kernels_fused.hip": ["-mllvm", "-sink-common-insts=false",
+                          "-mllvm", "-structurizecfg-skip-uniform-regions=true"],
 }
 
 

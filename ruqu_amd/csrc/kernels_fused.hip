@@ -59,6 +59,16 @@ __device__ __forceinline__ uint32_t lds_addr(uint32_t t) {
   else return (t ^ ((t >> 4) & 15u)) * 16u;
 }
 
+// Per-thread part of the LDS address, hidden from the optimizer: otherwise it
+// factors the common *8 out of (thread part ^ register part) and each access
+// pays a shift on top of the XOR.
+template <class T>
+__device__ __forceinline__ uint32_t lane_lds_addr(uint32_t t) {
+  uint32_t a = lds_addr<T>(t);
+  asm("" : "+v"(a));
+  return a;
+}
+
 template <class T>
 __device__ __forceinline__ void load_amp(const T* __restrict__ a, uint64_t g, Cx<T>& v) {
   using V = typename Vec2<T>::type;
@@ -75,18 +85,58 @@ __device__ __forceinline__ void store_amp(T* __restrict__ a, uint64_t g, const C
   reinterpret_cast<V*>(a)[g] = x;
 }
 
-// 2x2 on register bit B (CTL: only pairs whose k satisfies ctl_reg).
-template <int B, bool CTL, class T>
-__device__ __forceinline__ void u1(Cx<T> (&v)[FUSE_R], const Cx<T>* __restrict__ mp, uint32_t ctl_reg) {
+// Every op reads the 16 amplitudes from x and writes all 16 to y, a distinct
+// register set.  apply_ops alternates the two sets (x -> y, then y -> x), so
+// an op's results land in fresh registers and no copy back is needed between
+// ops; entries an op leaves unchanged cost one move.
+template <class T>
+using Amps = Cx<T>[FUSE_R];
+
+// a with the sign bits of both parts XORed with `sign` (0 or 0x80000000).
+template <class T>
+__device__ __forceinline__ Cx<T> flip(Cx<T> a, uint32_t sign) {
+  if constexpr (sizeof(T) == 4) {
+    return Cx<T>{__uint_as_float(__float_as_uint(a.re) ^ sign), __uint_as_float(__float_as_uint(a.im) ^ sign)};
+  } else {
+    const unsigned long long s = (unsigned long long)sign << 32;
+    return Cx<T>{__longlong_as_double(__double_as_longlong(a.re) ^ s),
+                 __longlong_as_double(__double_as_longlong(a.im) ^ s)};
+  }
+}
+
+// A copy of a in new registers.  Ops that pass an amplitude through unchanged
+// (permutations, partial sign flips, the padding record) copy it explicitly:
+// if an op's output were the input value itself, the register allocator would
+// merge the two register sets at the join of the op switch and then have to
+// copy the results of every other op instead.
+__device__ __forceinline__ uint32_t fresh32(uint32_t a) {
+  uint32_t r;
+  asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(a));
+  return r;
+}
+template <class T>
+__device__ __forceinline__ Cx<T> fresh(Cx<T> a) {
+  if constexpr (sizeof(T) == 4) {
+    return Cx<T>{__uint_as_float(fresh32(__float_as_uint(a.re))), __uint_as_float(fresh32(__float_as_uint(a.im)))};
+  } else {
+    auto f = [](double d) {
+      const unsigned long long u = __double_as_longlong(d);
+      return __longlong_as_double(((unsigned long long)fresh32((uint32_t)(u >> 32)) << 32) | fresh32((uint32_t)u));
+    };
+    return Cx<T>{f(a.re), f(a.im)};
+  }
+}
+
+// 2x2 on register bit B.
+template <int B, class T>
+__device__ __forceinline__ void u1(const Amps<T>& x, Amps<T>& y, const Cx<T>* __restrict__ mp) {
   const Cx<T> m00 = mp[0], m01 = mp[1], m10 = mp[2], m11 = mp[3];
 #pragma unroll
   for (int k = 0; k < FUSE_R; ++k) {
     if (k & (1 << B)) continue;
     const int k1 = k | (1 << B);
-    if (CTL && (k & ctl_reg) != ctl_reg) continue;
-    Cx<T> x = v[k], y = v[k1];
-    v[k] = cmadd(m00, x, m01, y);
-    v[k1] = cmadd(m10, x, m11, y);
+    y[k] = cmadd(m00, x[k], m01, x[k1]);
+    y[k1] = cmadd(m10, x[k], m11, x[k1]);
   }
 }
 
@@ -97,80 +147,95 @@ __device__ __forceinline__ Cx<T> row4(const Cx<T>* __restrict__ mp, int r, Cx<T>
   Cx<T> s2 = cmadd(mp[r * 4 + 2], x2, mp[r * 4 + 3], x3);
   return Cx<T>{s.re + s2.re, s.im + s2.im};
 }
-
-template <int B0, int B1, int K, class T>
-__device__ __forceinline__ void u2_one(Cx<T> (&v)[FUSE_R], const Cx<T>* __restrict__ mp) {
-  if constexpr ((K & ((1 << B0) | (1 << B1))) == 0) {
-    constexpr int i0 = K, i1 = K | (1 << B1), i2 = K | (1 << B0), i3 = K | (1 << B0) | (1 << B1);
-    const Cx<T> x0 = v[i0], x1 = v[i1], x2 = v[i2], x3 = v[i3];
-    v[i0] = row4(mp, 0, x0, x1, x2, x3);
-    v[i1] = row4(mp, 1, x0, x1, x2, x3);
-    v[i2] = row4(mp, 2, x0, x1, x2, x3);
-    v[i3] = row4(mp, 3, x0, x1, x2, x3);
-  }
-}
-
-template <int B0, int B1, class T, int... K>
-__device__ __forceinline__ void u2_all(Cx<T> (&v)[FUSE_R], const Cx<T>* __restrict__ mp,
-                                       std::integer_sequence<int, K...>) {
-  (u2_one<B0, B1, K>(v, mp), ...);
-}
-
 template <int B0, int B1, class T>
-__device__ __forceinline__ void u2(Cx<T> (&v)[FUSE_R], const Cx<T>* __restrict__ mp) {
-  u2_all<B0, B1>(v, mp, std::make_integer_sequence<int, FUSE_R>{});
-}
-
-// Controlled swap on register bit B (CNOT target).
-template <int B, class T>
-__device__ __forceinline__ void cx(Cx<T> (&v)[FUSE_R], uint32_t ctl_reg) {
+__device__ __forceinline__ void u2(const Amps<T>& x, Amps<T>& y, const Cx<T>* __restrict__ mp) {
 #pragma unroll
   for (int k = 0; k < FUSE_R; ++k) {
-    if (k & (1 << B)) continue;
-    const int k1 = k | (1 << B);
-    if ((k & ctl_reg) != ctl_reg) continue;
-    const Cx<T> x = v[k];
-    v[k] = v[k1];
-    v[k1] = x;
+    if (k & ((1 << B0) | (1 << B1))) continue;
+    const int i0 = k, i1 = k | (1 << B1), i2 = k | (1 << B0), i3 = k | (1 << B0) | (1 << B1);
+    y[i0] = row4(mp, 0, x[i0], x[i1], x[i2], x[i3]);
+    y[i1] = row4(mp, 1, x[i0], x[i1], x[i2], x[i3]);
+    y[i2] = row4(mp, 2, x[i0], x[i1], x[i2], x[i3]);
+    y[i3] = row4(mp, 3, x[i0], x[i1], x[i2], x[i3]);
   }
 }
 
+// CNOT target on register bit B: with register control bit C, unconditional,
+// or under a per-thread / per-tile condition c.
+template <int B, int C, class T>
+__device__ __forceinline__ void cx_rc(const Amps<T>& x, Amps<T>& y) {
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) y[k] = fresh(((k >> C) & 1) ? x[k ^ (1 << B)] : x[k]);
+}
+template <int B, class T>
+__device__ __forceinline__ void cx_all(const Amps<T>& x, Amps<T>& y) {
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) y[k] = fresh(x[k ^ (1 << B)]);
+}
+// Per lane: b where the lane's bit of `lanes` is set, else a.  One
+// v_cndmask per 32-bit word: a plain conditional between two entries of the
+// register array would be folded into an indexed access to it, which puts the
+// array in scratch memory.
+__device__ __forceinline__ uint32_t pick32(uint32_t a, uint32_t b, uint64_t lanes) {
+  uint32_t r;
+  asm("v_cndmask_b32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(lanes));
+  return r;
+}
+__device__ __forceinline__ float pick(float a, float b, uint64_t lanes) {
+  return __uint_as_float(pick32(__float_as_uint(a), __float_as_uint(b), lanes));
+}
+__device__ __forceinline__ double pick(double a, double b, uint64_t lanes) {
+  const unsigned long long ua = __double_as_longlong(a), ub = __double_as_longlong(b);
+  const uint32_t lo = pick32((uint32_t)ua, (uint32_t)ub, lanes), hi = pick32((uint32_t)(ua >> 32), (uint32_t)(ub >> 32), lanes);
+  return __longlong_as_double(((unsigned long long)hi << 32) | lo);
+}
+template <int B, class T>
+__device__ __forceinline__ void cx_sel(const Amps<T>& x, Amps<T>& y, uint64_t lanes) {
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) {
+    const Cx<T> a = x[k], b = x[k ^ (1 << B)];
+    y[k] = Cx<T>{pick(a.re, b.re, lanes), pick(a.im, b.im, lanes)};
+  }
+}
+
+// CZ: negate where register bits P0 and P1 are set; where register bit P is
+// set (sign applied: the other bit's value << 31); everywhere (sign).
+template <int P0, int P1, class T>
+__device__ __forceinline__ void cz_rr(const Amps<T>& x, Amps<T>& y) {
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) y[k] = (((k >> P0) & (k >> P1)) & 1) ? flip(x[k], 0x80000000u) : fresh(x[k]);
+}
+template <int P, class T>
+__device__ __forceinline__ void cz_r(const Amps<T>& x, Amps<T>& y, uint32_t sign) {
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) y[k] = ((k >> P) & 1) ? flip(x[k], sign) : fresh(x[k]);
+}
 template <class T>
-__device__ __forceinline__ void dispatch_cx(Cx<T> (&v)[FUSE_R], int b, uint32_t ctl_reg) {
-  switch (b) {
-    case 0: cx<0>(v, ctl_reg); break;
-    case 1: cx<1>(v, ctl_reg); break;
-    case 2: cx<2>(v, ctl_reg); break;
-    default: cx<3>(v, ctl_reg); break;
-  }
+__device__ __forceinline__ void flip_all(const Amps<T>& x, Amps<T>& y, uint32_t sign) {
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) y[k] = flip(x[k], sign);
 }
 
-template <bool CTL, class T>
-__device__ __forceinline__ void dispatch_u1(Cx<T> (&v)[FUSE_R], int b, const Cx<T>* mp, uint32_t ctl_reg) {
-  switch (b) {
-    case 0: u1<0, CTL>(v, mp, ctl_reg); break;
-    case 1: u1<1, CTL>(v, mp, ctl_reg); break;
-    case 2: u1<2, CTL>(v, mp, ctl_reg); break;
-    default: u1<3, CTL>(v, mp, ctl_reg); break;
+// Diagonal factors: selected by register bit P; indexed by register bits
+// (P0, P1); one factor for all 16 amplitudes.
+template <int P, class T>
+__device__ __forceinline__ void dsel_r(const Amps<T>& x, Amps<T>& y, Cx<T> f0, Cx<T> f1) {
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) y[k] = cmul(((k >> P) & 1) ? f1 : f0, x[k]);
+}
+template <int P0, int P1, class T>
+__device__ __forceinline__ void d2_rr(const Amps<T>& x, Amps<T>& y, const Cx<T>* __restrict__ mp) {
+  const Cx<T> f0 = mp[0], f1 = mp[1], f2 = mp[2], f3 = mp[3];
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) {
+    const int i = ((k >> P0) & 1) * 2 + ((k >> P1) & 1);
+    y[k] = cmul(i == 0 ? f0 : (i == 1 ? f1 : (i == 2 ? f2 : f3)), x[k]);
   }
 }
-
 template <class T>
-__device__ __forceinline__ void dispatch_u2(Cx<T> (&v)[FUSE_R], int b0, int b1, const Cx<T>* mp) {
-  switch (b0 * 4 + b1) {
-    case 1: u2<0, 1>(v, mp); break;
-    case 2: u2<0, 2>(v, mp); break;
-    case 3: u2<0, 3>(v, mp); break;
-    case 4: u2<1, 0>(v, mp); break;
-    case 6: u2<1, 2>(v, mp); break;
-    case 7: u2<1, 3>(v, mp); break;
-    case 8: u2<2, 0>(v, mp); break;
-    case 9: u2<2, 1>(v, mp); break;
-    case 11: u2<2, 3>(v, mp); break;
-    case 12: u2<3, 0>(v, mp); break;
-    case 13: u2<3, 1>(v, mp); break;
-    default: u2<3, 2>(v, mp); break;
-  }
+__device__ __forceinline__ void dall(const Amps<T>& x, Amps<T>& y, Cx<T> f) {
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) y[k] = cmul(f, x[k]);
 }
 
 // Packed op record (built from FOp at launch, one 8-byte scalar load per op):
@@ -182,68 +247,11 @@ struct POp {
   uint32_t x, y;
 };
 constexpr int PL_NONE = 0, PL_REG = LOC_REG + 1, PL_THR = LOC_THR + 1, PL_OUT = LOC_OUT + 1;
+constexpr int POP_NOP = 6;  // kind of the padding record: copies the amplitudes
 
 // Value of a non-register location bit for this thread / tile.
 __device__ __forceinline__ uint32_t loc_val(int loc, int pos, uint32_t tid) {
   return loc == PL_THR ? ((tid >> pos) & 1u) : ((blockIdx.x >> pos) & 1u);
-}
-
-template <class T>
-__device__ __forceinline__ Cx<T> neg_if(Cx<T> a, bool c) {
-  return c ? Cx<T>{-a.re, -a.im} : a;
-}
-
-// Controlled swap on register bit B with register control bit C.
-template <int B, int C, class T>
-__device__ __forceinline__ void cx_rc(Cx<T> (&v)[FUSE_R]) {
-#pragma unroll
-  for (int k = 0; k < FUSE_R; ++k) {
-    if ((k & (1 << B)) || !(k & (1 << C))) continue;
-    const Cx<T> x = v[k];
-    v[k] = v[k | (1 << B)];
-    v[k | (1 << B)] = x;
-  }
-}
-template <int B, class T>
-__device__ __forceinline__ void cx_all(Cx<T> (&v)[FUSE_R]) {
-#pragma unroll
-  for (int k = 0; k < FUSE_R; ++k) {
-    if (k & (1 << B)) continue;
-    const Cx<T> x = v[k];
-    v[k] = v[k | (1 << B)];
-    v[k | (1 << B)] = x;
-  }
-}
-
-// CZ with both bits in registers: negate where bits P0 and P1 of k are set.
-template <int P0, int P1, class T>
-__device__ __forceinline__ void cz_rr(Cx<T> (&v)[FUSE_R]) {
-#pragma unroll
-  for (int k = 0; k < FUSE_R; ++k)
-    if (((k >> P0) & (k >> P1)) & 1) v[k] = Cx<T>{-v[k].re, -v[k].im};
-}
-// CZ with one register bit P: negate where bit P of k is set and c holds.
-template <int P, class T>
-__device__ __forceinline__ void cz_r(Cx<T> (&v)[FUSE_R], bool c) {
-#pragma unroll
-  for (int k = 0; k < FUSE_R; ++k)
-    if ((k >> P) & 1) v[k] = neg_if(v[k], c);
-}
-// Diagonal 2-entry factor selected by register bit P.
-template <int P, class T>
-__device__ __forceinline__ void dsel_r(Cx<T> (&v)[FUSE_R], Cx<T> f0, Cx<T> f1) {
-#pragma unroll
-  for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(((k >> P) & 1) ? f1 : f0, v[k]);
-}
-// Diagonal 4-entry factor indexed by register bits (P0, P1).
-template <int P0, int P1, class T>
-__device__ __forceinline__ void d2_rr(Cx<T> (&v)[FUSE_R], const Cx<T>* __restrict__ mp) {
-  const Cx<T> f0 = mp[0], f1 = mp[1], f2 = mp[2], f3 = mp[3];
-#pragma unroll
-  for (int k = 0; k < FUSE_R; ++k) {
-    const int i = ((k >> P0) & 1) * 2 + ((k >> P1) & 1);
-    v[k] = cmul(i == 0 ? f0 : (i == 1 ? f1 : (i == 2 ? f2 : f3)), v[k]);
-  }
 }
 
 #define RQ_P2(F, ...)                      \
@@ -269,76 +277,84 @@ __device__ __forceinline__ void d2_rr(Cx<T> (&v)[FUSE_R], const Cx<T>* __restric
     default: F<3>(__VA_ARGS__); break;   \
   }
 
+// One op, x -> y.  U1 ops carry no controls (the planner folds only
+// uncontrolled 1-qubit gates into them).
 template <class T>
-__device__ __forceinline__ void apply_cx(Cx<T> (&v)[FUSE_R], int b, int cl, int cp, uint32_t tid) {
-  if (cl == PL_REG) {
-    const int p0 = b, p1 = cp;
-    RQ_P2(cx_rc, v)
-  } else if (cl == PL_NONE || loc_val(cl, cp, tid)) {
-    RQ_P1(cx_all, b, v)
+__device__ __forceinline__ void apply_one(const Amps<T>& x, Amps<T>& y, uint32_t w, const Cx<T>* __restrict__ mp,
+                                          uint32_t tid) {
+  const int kind = w & 7u, t0 = (w >> 3) & 3u, t1 = (w >> 5) & 3u, cl = (w >> 7) & 3u, cp = (w >> 9) & 31u,
+            dl0 = (w >> 14) & 3u, dp0 = (w >> 16) & 63u, dl1 = (w >> 22) & 3u, dp1 = (w >> 24) & 63u;
+  switch (kind) {
+    case FOP_U1:
+      RQ_P1(u1, t0, x, y, mp)
+      break;
+    case FOP_U2: {
+      const int p0 = t0, p1 = t1;
+      RQ_P2(u2, x, y, mp)
+      break;
+    }
+    case FOP_CX:
+      if (cl == PL_REG) {
+        const int p0 = t0, p1 = cp;
+        RQ_P2(cx_rc, x, y)
+      } else if (cl == PL_NONE) {
+        RQ_P1(cx_all, t0, x, y)
+      } else {
+        RQ_P1(cx_sel, t0, x, y, __builtin_amdgcn_ballot_w64(loc_val(cl, cp, tid) != 0u))
+      }
+      break;
+    case FOP_CZ:
+      if (dl0 == PL_REG && dl1 == PL_REG) {
+        const int p0 = dp0, p1 = dp1;
+        RQ_P2(cz_rr, x, y)
+      } else if (dl0 == PL_REG) {
+        RQ_P1(cz_r, dp0, x, y, loc_val(dl1, dp1, tid) << 31)
+      } else if (dl1 == PL_REG) {
+        RQ_P1(cz_r, dp1, x, y, loc_val(dl0, dp0, tid) << 31)
+      } else {
+        flip_all(x, y, (loc_val(dl0, dp0, tid) & loc_val(dl1, dp1, tid)) << 31);
+      }
+      break;
+    case FOP_D1:
+      if (dl0 == PL_REG) {
+        RQ_P1(dsel_r, dp0, x, y, mp[0], mp[1])
+      } else {
+        dall(x, y, mp[loc_val(dl0, dp0, tid)]);
+      }
+      break;
+    case POP_NOP:
+#pragma unroll
+      for (int k = 0; k < FUSE_R; ++k) y[k] = fresh(x[k]);
+      break;
+    default:  // FOP_D2
+      if (dl0 == PL_REG && dl1 == PL_REG) {
+        const int p0 = dp0, p1 = dp1;
+        RQ_P2(d2_rr, x, y, mp)
+      } else if (dl0 == PL_REG) {
+        const uint32_t c1 = loc_val(dl1, dp1, tid);
+        RQ_P1(dsel_r, dp0, x, y, mp[c1], mp[2 + c1])
+      } else if (dl1 == PL_REG) {
+        const uint32_t c0 = loc_val(dl0, dp0, tid);
+        RQ_P1(dsel_r, dp1, x, y, mp[2 * c0], mp[2 * c0 + 1])
+      } else {
+        dall(x, y, mp[2 * loc_val(dl0, dp0, tid) + loc_val(dl1, dp1, tid)]);
+      }
+      break;
   }
 }
 
+// Ops [b, e) of a phase, two per iteration (v -> w -> v).  Every phase holds
+// an even number of records (pack_ops pads odd ones with POP_NOP): an odd
+// tail would merge the two register sets after the loop, and the compiler
+// then copies all 16 amplitudes on every iteration.
 template <class T>
-__device__ __forceinline__ void apply_ops(Cx<T> (&v)[FUSE_R], const POp* __restrict__ ops, int b, int e,
+__device__ __forceinline__ void apply_ops(Amps<T>& v, const POp* __restrict__ ops, int b, int e,
                                           const Cx<T>* __restrict__ pool, uint32_t tid) {
-  for (int o = b; o < e; ++o) {
-    const POp w = ops[o];
-    const int kind = w.x & 7u, t0 = (w.x >> 3) & 3u, t1 = (w.x >> 5) & 3u, cl = (w.x >> 7) & 3u,
-              cp = (w.x >> 9) & 31u, dl0 = (w.x >> 14) & 3u, dp0 = (w.x >> 16) & 63u, dl1 = (w.x >> 22) & 3u,
-              dp1 = (w.x >> 24) & 63u;
-    const Cx<T>* mp = pool + w.y;
-    switch (kind) {
-      case FOP_U1:
-        if (cl == PL_NONE) dispatch_u1<false>(v, t0, mp, 0u);
-        else if (cl == PL_REG) dispatch_u1<true>(v, t0, mp, 1u << cp);
-        else if (loc_val(cl, cp, tid)) dispatch_u1<false>(v, t0, mp, 0u);
-        break;
-      case FOP_U2:
-        dispatch_u2(v, t0, t1, mp);
-        break;
-      case FOP_CX:
-        apply_cx(v, t0, cl, cp, tid);
-        break;
-      case FOP_CZ:
-        if (dl0 == PL_REG && dl1 == PL_REG) {
-          const int p0 = dp0, p1 = dp1;
-          RQ_P2(cz_rr, v)
-        } else if (dl0 == PL_REG) {
-          RQ_P1(cz_r, dp0, v, loc_val(dl1, dp1, tid) != 0u)
-        } else if (dl1 == PL_REG) {
-          RQ_P1(cz_r, dp1, v, loc_val(dl0, dp0, tid) != 0u)
-        } else if (loc_val(dl0, dp0, tid) & loc_val(dl1, dp1, tid)) {
-#pragma unroll
-          for (int k = 0; k < FUSE_R; ++k) v[k] = Cx<T>{-v[k].re, -v[k].im};
-        }
-        break;
-      case FOP_D1:
-        if (dl0 == PL_REG) {
-          RQ_P1(dsel_r, dp0, v, mp[0], mp[1])
-        } else {
-          const Cx<T> f = mp[loc_val(dl0, dp0, tid)];
-#pragma unroll
-          for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(f, v[k]);
-        }
-        break;
-      default:  // FOP_D2
-        if (dl0 == PL_REG && dl1 == PL_REG) {
-          const int p0 = dp0, p1 = dp1;
-          RQ_P2(d2_rr, v, mp)
-        } else if (dl0 == PL_REG) {
-          const uint32_t c1 = loc_val(dl1, dp1, tid);
-          RQ_P1(dsel_r, dp0, v, mp[c1], mp[2 + c1])
-        } else if (dl1 == PL_REG) {
-          const uint32_t c0 = loc_val(dl0, dp0, tid);
-          RQ_P1(dsel_r, dp1, v, mp[2 * c0], mp[2 * c0 + 1])
-        } else {
-          const Cx<T> f = mp[2 * loc_val(dl0, dp0, tid) + loc_val(dl1, dp1, tid)];
-#pragma unroll
-          for (int k = 0; k < FUSE_R; ++k) v[k] = cmul(f, v[k]);
-        }
-        break;
-    }
+  Cx<T> w[FUSE_R];
+  for (int o = b; o < e; o += 2) {
+    const POp p0 = ops[o], p1 = ops[o + 1];
+    apply_one(v, w, p0.x, pool + p0.y, tid);
+    apply_one(w, v, p1.x, pool + p1.y, tid);
   }
 }
 
@@ -439,8 +455,9 @@ template <class T>
 __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FStage* __restrict__ stp,
                                                       const FPhase* __restrict__ phases, const POp* __restrict__ ops,
                                                       const Cx<T>* __restrict__ pool) {
-  extern __shared__ unsigned char smem_raw[];
-  Cx<T>* lds = reinterpret_cast<Cx<T>*>(smem_raw);
+  // A static array (the kernel's only LDS) sits at LDS address 0, so the
+  // compiler folds its base into the addresses.
+  __shared__ Cx<T> lds[1 << FUSE_MAX_M];
   const FStage& st = *stp;
   const uint32_t tid = threadIdx.x;
   uint64_t outer = 0;
@@ -459,13 +476,13 @@ __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FS
       } else {
         const Map nat = make_natural_map(st, tid, outer);
         Unroll<FUSE_R, T>::gather_g(v, a, nat);
-        Unroll<FUSE_R, T>::scatter_l(v, lds, nat, lds_addr<T>(nat.tthr));
+        Unroll<FUSE_R, T>::scatter_l(v, lds, nat, lane_lds_addr<T>(nat.tthr));
         __syncthreads();
-        Unroll<FUSE_R, T>::gather_l(v, lds, mp, lds_addr<T>(mp.tthr));
+        Unroll<FUSE_R, T>::gather_l(v, lds, mp, lane_lds_addr<T>(mp.tthr));
         __syncthreads();
       }
     } else {
-      Unroll<FUSE_R, T>::gather_l(v, lds, mp, lds_addr<T>(mp.tthr));
+      Unroll<FUSE_R, T>::gather_l(v, lds, mp, lane_lds_addr<T>(mp.tthr));
       __syncthreads();
     }
     apply_ops(v, ops, ph.op_begin, ph.op_end, pool, tid);
@@ -474,22 +491,26 @@ __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FS
         Unroll<FUSE_R, T>::scatter_g(v, a, mp);
       } else {
         const Map nat = make_natural_map(st, tid, outer);
-        Unroll<FUSE_R, T>::scatter_l(v, lds, mp, lds_addr<T>(mp.tthr));
+        Unroll<FUSE_R, T>::scatter_l(v, lds, mp, lane_lds_addr<T>(mp.tthr));
         __syncthreads();
-        Unroll<FUSE_R, T>::gather_l(v, lds, nat, lds_addr<T>(nat.tthr));
+        Unroll<FUSE_R, T>::gather_l(v, lds, nat, lane_lds_addr<T>(nat.tthr));
         Unroll<FUSE_R, T>::scatter_g(v, a, nat);
       }
     } else {
-      Unroll<FUSE_R, T>::scatter_l(v, lds, mp, lds_addr<T>(mp.tthr));
+      Unroll<FUSE_R, T>::scatter_l(v, lds, mp, lane_lds_addr<T>(mp.tthr));
       __syncthreads();
     }
   }
 }
 
 // Packs the plan's ops for the kernel: one 8-byte record each, control and
-// diagonal positions of outer qubits turned into tile-id (blockIdx) bits.
-int pack_ops(const FusedPlan& plan, std::vector<POp>& out) {
-  out.assign(plan.ops.size(), POp{0u, 0u});
+// diagonal positions of outer qubits turned into tile-id (blockIdx) bits, and
+// every phase padded to an even record count with POP_NOP.  `phases` receives
+// the plan's phases with op ranges into `out`.
+int pack_ops(const FusedPlan& plan, std::vector<POp>& out, std::vector<FPhase>& phases) {
+  out.clear();
+  out.reserve(plan.ops.size() + plan.phases.size());
+  phases = plan.phases;
   for (const FStage& st : plan.stages) {
     auto tile_bit = [&](int q) {
       for (int j = 0; j < st.nouter; ++j)
@@ -507,10 +528,13 @@ int pack_ops(const FusedPlan& plan, std::vector<POp>& out) {
       return pos >= 0 && pos < 32;
     };
     for (int ph = st.phase0; ph < st.phase0 + st.nphase; ++ph) {
+      phases[ph].op_begin = (int32_t)out.size();
       for (int o = plan.phases[ph].op_begin; o < plan.phases[ph].op_end; ++o) {
         const FOp& op = plan.ops[o];
         uint32_t cl = 0, cp = 0, dl0 = 0, dp0 = 0, dl1 = 0, dp1 = 0;
-        if (op.kind == FOP_U1 || op.kind == FOP_CX) {
+        if (op.kind == FOP_U1 && (op.ctl_reg || op.ctl_thr || op.ctl_out))
+          return fail(RQ_ERR_ARGUMENT, "fused 1-qubit op with a control");
+        if (op.kind == FOP_CX) {
           const int nctl = __builtin_popcount(op.ctl_reg) + __builtin_popcount(op.ctl_thr) +
                            __builtin_popcountll(op.ctl_out);
           if (nctl > 1) return fail(RQ_ERR_ARGUMENT, "fused op with more than one control");
@@ -528,11 +552,14 @@ int pack_ops(const FusedPlan& plan, std::vector<POp>& out) {
           if (op.kind != FOP_D1 && !enc(op.dloc1, op.dpos1, &dl1, &dp1))
             return fail(RQ_ERR_ARGUMENT, "fused diagonal bit not located");
         }
-        POp& w = out[o];
+        POp w;
         w.x = (uint32_t)op.kind | ((uint32_t)op.t0 << 3) | ((uint32_t)op.t1 << 5) | (cl << 7) | (cp << 9) |
               (dl0 << 14) | (dp0 << 16) | (dl1 << 22) | (dp1 << 24);
         w.y = (uint32_t)op.mat;
+        out.push_back(w);
       }
+      if ((out.size() - (size_t)phases[ph].op_begin) & 1) out.push_back(POp{(uint32_t)POP_NOP, 0u});
+      phases[ph].op_end = (int32_t)out.size();
     }
   }
   return RQ_OK;
@@ -541,8 +568,9 @@ int pack_ops(const FusedPlan& plan, std::vector<POp>& out) {
 template <class T>
 int launch_plan_t(State& s, const FusedPlan& plan) {
   std::vector<POp> pops;
-  if (int rc = pack_ops(plan, pops)) return rc;
-  const size_t nst = plan.stages.size(), nph = plan.phases.size(), nop = plan.ops.size(), npool = plan.pool.size();
+  std::vector<FPhase> phases;
+  if (int rc = pack_ops(plan, pops, phases)) return rc;
+  const size_t nst = plan.stages.size(), nph = phases.size(), nop = pops.size(), npool = plan.pool.size();
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t o_st = 0, o_ph = al(nst * sizeof(FStage)), o_op = o_ph + al(nph * sizeof(FPhase)),
                o_pool = o_op + al(nop * sizeof(POp)), total = o_pool + al(npool * sizeof(Cx<T>));
@@ -559,26 +587,19 @@ int launch_plan_t(State& s, const FusedPlan& plan) {
   }
   std::vector<unsigned char> host(total, 0);
   std::memcpy(host.data() + o_st, plan.stages.data(), nst * sizeof(FStage));
-  std::memcpy(host.data() + o_ph, plan.phases.data(), nph * sizeof(FPhase));
+  std::memcpy(host.data() + o_ph, phases.data(), nph * sizeof(FPhase));
   std::memcpy(host.data() + o_op, pops.data(), nop * sizeof(POp));
   Cx<T>* pp = reinterpret_cast<Cx<T>*>(host.data() + o_pool);
   for (size_t k = 0; k < npool; ++k) pp[k] = Cx<T>{(T)plan.pool[k].re, (T)plan.pool[k].im};
   // The host buffer is pageable: wait for the copy before it goes out of scope.
   RQ_HIP(hipMemcpyAsync(s.prog, host.data(), total, hipMemcpyHostToDevice, s.stream));
   RQ_HIP(hipStreamSynchronize(s.stream));
-  static bool attr_set = false;
-  if (!attr_set) {
-    RQ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused_stage<T>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    attr_set = true;
-  }
   unsigned char* base = reinterpret_cast<unsigned char*>(s.prog);
   for (size_t i = 0; i < nst; ++i) {
     const FStage& st = plan.stages[i];
     const unsigned threads = 1u << (st.m - FUSE_R_BITS);
     const uint64_t tiles = 1ull << st.nouter;
-    const size_t lds = (size_t(1) << st.m) * sizeof(Cx<T>);
-    hipLaunchKernelGGL(k_fused_stage<T>, dim3((unsigned)tiles), dim3(threads), lds, s.stream, (T*)s.amps,
+    hipLaunchKernelGGL(k_fused_stage<T>, dim3((unsigned)tiles), dim3(threads), 0, s.stream, (T*)s.amps,
                        reinterpret_cast<const FStage*>(base + o_st + i * sizeof(FStage)),
                        reinterpret_cast<const FPhase*>(base + o_ph), reinterpret_cast<const POp*>(base + o_op),
                        reinterpret_cast<const Cx<T>*>(base + o_pool));
@@ -592,8 +613,7 @@ int launch_plan_t(State& s, const FusedPlan& plan) {
 int launch_fused_plan(State& s, const FusedPlan& plan) {
   if (plan.stages.empty()) return RQ_OK;
   for (const FStage& st : plan.stages)
-    if (st.m < FUSE_R_BITS || st.m > FUSE_MAX_M || st.nouter > 31 ||
-        (size_t(1) << st.m) * s.amp_bytes() > 160 * 1024)
+    if (st.m < FUSE_R_BITS || st.m > FUSE_MAX_M || st.nouter > 31)
       return fail(RQ_ERR_ARGUMENT, "fused stage shape out of range");
   if (s.prec == RQ_C64) return launch_plan_t<double>(s, plan);
   return launch_plan_t<float>(s, plan);
