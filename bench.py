@@ -233,7 +233,7 @@ def main():
 
     value = ngates * args.steps / el
     line = {
-        "metric": "gate-apps/s (30q c32 random circuit, depth 40)",
+        "metric": "gate-apps/s (%dq %s random circuit, depth %d)" % (nq, prec, args.depth),
         "value": round(value, 2),
         "unit": "gate-apps/s",
         "n_gpus": 1,

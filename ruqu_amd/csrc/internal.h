@@ -22,6 +22,18 @@ int hip_fail(hipError_t e, const char* what);
     if (_e != hipSuccess) return ::rq::hip_fail(_e, #call); \
   } while (0)
 
+// One grid dimension may cover at most 2^32 - 1 work-items, which a
+// one-thread-per-amplitude launch exceeds from 32 qubits on.  Element-wise
+// launches therefore put at most 2^16 workgroups in a row and the rows in
+// gridDim.y; kernels take their workgroup index from flat_block() and keep
+// their bounds checks (the last row may run past the end).
+constexpr uint64_t GRID_ROW = 1ull << 16;
+inline dim3 grid_blocks(uint64_t blocks) {
+  if (blocks <= GRID_ROW) return dim3((unsigned)(blocks ? blocks : 1));
+  return dim3((unsigned)GRID_ROW, (unsigned)((blocks + GRID_ROW - 1) / GRID_ROW));
+}
+__device__ __forceinline__ uint64_t flat_block() { return (uint64_t)blockIdx.y * gridDim.x + blockIdx.x; }
+
 // Device-resident state vector.
 struct State {
   uint32_t nq = 0;

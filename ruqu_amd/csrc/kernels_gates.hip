@@ -72,7 +72,7 @@ __device__ __forceinline__ uint64_t ins0(uint64_t p, uint32_t q) {
 template <int U, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_1q_f32_v2(float4* __restrict__ a, uint32_t q, uint64_t ngroups,
                                                      M2<float> m) {
-  const uint64_t g0 = (uint64_t)blockIdx.x * U * BLOCK + threadIdx.x;
+  const uint64_t g0 = flat_block() * U * BLOCK + threadIdx.x;
   float4 x[U], y[U];
   uint64_t ii[U], jj[U];
 #pragma unroll
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(BLOCK) void k_1q_f32_v2(float4* __restrict__ a, uin
 // ---- 1q, f32, q == 0: one float4 holds the pair (a, b) --------------------
 template <int U, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_1q_f32_q0(float4* __restrict__ a, uint64_t npairs, M2<float> m) {
-  const uint64_t g0 = (uint64_t)blockIdx.x * U * BLOCK + threadIdx.x;
+  const uint64_t g0 = flat_block() * U * BLOCK + threadIdx.x;
   float4 x[U];
 #pragma unroll
   for (int k = 0; k < U; ++k) {
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(BLOCK) void k_1q_f32_q0(float4* __restrict__ a, uin
 template <int U, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_1q_f64(double2* __restrict__ a, uint32_t q, uint64_t npairs,
                                                   M2<double> m) {
-  const uint64_t g0 = (uint64_t)blockIdx.x * U * BLOCK + threadIdx.x;
+  const uint64_t g0 = flat_block() * U * BLOCK + threadIdx.x;
   double2 x[U], y[U];
   uint64_t ii[U];
 #pragma unroll
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(BLOCK) void k_1q_f64(double2* __restrict__ a, uint3
 template <class T, class V>
 __global__ __launch_bounds__(BLOCK) void k_2q_dense(V* __restrict__ a, uint32_t q1, uint32_t q2, uint64_t nquads,
                                                     M4<T> m) {
-  uint64_t g = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t g = flat_block() * BLOCK + threadIdx.x;
   if (g >= nquads) return;
   uint32_t lo = q1 < q2 ? q1 : q2, hi = q1 < q2 ? q2 : q1;
   uint64_t base = ins0(ins0(g, lo), hi);
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(BLOCK) void k_2q_dense(V* __restrict__ a, uint32_t 
 // ---- CNOT(c, t): swap the t=0 / t=1 amplitudes where c=1 ------------------
 template <class V>
 __global__ __launch_bounds__(BLOCK) void k_cnot(V* __restrict__ a, uint32_t c, uint32_t t, uint64_t nquads) {
-  uint64_t g = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t g = flat_block() * BLOCK + threadIdx.x;
   if (g >= nquads) return;
   uint32_t lo = c < t ? c : t, hi = c < t ? t : c;
   uint64_t base = ins0(ins0(g, lo), hi) | (1ull << c);
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(BLOCK) void k_cnot(V* __restrict__ a, uint32_t c, u
 // ---- SWAP(q1, q2): exchange |01> and |10> ---------------------------------
 template <class V>
 __global__ __launch_bounds__(BLOCK) void k_swap(V* __restrict__ a, uint32_t q1, uint32_t q2, uint64_t nquads) {
-  uint64_t g = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t g = flat_block() * BLOCK + threadIdx.x;
   if (g >= nquads) return;
   uint32_t lo = q1 < q2 ? q1 : q2, hi = q1 < q2 ? q2 : q1;
   uint64_t base = ins0(ins0(g, lo), hi);
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(BLOCK) void k_swap(V* __restrict__ a, uint32_t q1, 
 // ---- CZ: (-1 + 0i) * a on |11> --------------------------------------------
 template <class T, class V>
 __global__ __launch_bounds__(BLOCK) void k_cz(V* __restrict__ a, uint32_t q1, uint32_t q2, uint64_t nquads) {
-  uint64_t g = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t g = flat_block() * BLOCK + threadIdx.x;
   if (g >= nquads) return;
   uint32_t lo = q1 < q2 ? q1 : q2, hi = q1 < q2 ? q2 : q1;
   uint64_t i = ins0(ins0(g, lo), hi) | (1ull << q1) | (1ull << q2);
@@ -222,7 +222,7 @@ __global__ __launch_bounds__(BLOCK) void k_cz(V* __restrict__ a, uint32_t q1, ui
 // ---- diagonal 2q (Rzz): a_i *= d[b1*2 + b2], f32 float4 / f64 double2 ----
 __global__ __launch_bounds__(BLOCK) void k_diag2_f32(float4* __restrict__ a, uint32_t q1, uint32_t q2,
                                                      uint64_t nvec, M4<float> d) {
-  uint64_t g = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t g = flat_block() * BLOCK + threadIdx.x;
   if (g >= nvec) return;
   float4 x = a[g];
   uint64_t i0 = 2 * g, i1 = 2 * g + 1;
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(BLOCK) void k_diag2_f32(float4* __restrict__ a, uin
 
 __global__ __launch_bounds__(BLOCK) void k_diag2_f64(double2* __restrict__ a, uint32_t q1, uint32_t q2, uint64_t n,
                                                      M4<double> d) {
-  uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t i = flat_block() * BLOCK + threadIdx.x;
   if (i >= n) return;
   double2 x = a[i];
   int k = (int)(((i >> q1) & 1) * 2 + ((i >> q2) & 1));
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(BLOCK) void k_diag2_f64(double2* __restrict__ a, ui
 // ---- |0..0> -----------------------------------------------------------------
 template <class V>
 __global__ __launch_bounds__(BLOCK) void k_init_zero(V* __restrict__ a, uint64_t n) {
-  uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t i = flat_block() * BLOCK + threadIdx.x;
   if (i >= n) return;
   V z{};
   if (i == 0) z.x = 1;
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(BLOCK) void k_init_zero(V* __restrict__ a, uint64_t
 // ---- measurement collapse (state.rs:310-317, mixed_precision.rs:499-506) ---
 template <class T, class V>
 __global__ __launch_bounds__(BLOCK) void k_collapse(V* __restrict__ a, uint32_t q, int result, T nf, uint64_t n) {
-  uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t i = flat_block() * BLOCK + threadIdx.x;
   if (i >= n) return;
   V x = a[i];
   int one = (int)((i >> q) & 1);
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(BLOCK) void k_collapse(V* __restrict__ a, uint32_t 
 // ---- scale every amplitude (normalize, state.rs:419-421) -------------------
 template <class T, class V>
 __global__ __launch_bounds__(BLOCK) void k_scale(V* __restrict__ a, T f, uint64_t n) {
-  uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t i = flat_block() * BLOCK + threadIdx.x;
   if (i >= n) return;
   V x = a[i];
   x.x = x.x * f;
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(BLOCK) void k_scale(V* __restrict__ a, T f, uint64_
 template <int U, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_copy16(float4* __restrict__ dst, const float4* __restrict__ src,
                                                   uint64_t nvec) {
-  const uint64_t g0 = (uint64_t)blockIdx.x * U * BLOCK + threadIdx.x;
+  const uint64_t g0 = flat_block() * U * BLOCK + threadIdx.x;
   float4 x[U];
 #pragma unroll
   for (int k = 0; k < U; ++k) {
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(BLOCK) void k_copy16(float4* __restrict__ dst, cons
   }
 }
 
-inline dim3 grid_for(uint64_t items, uint64_t per_block) { return dim3((unsigned)((items + per_block - 1) / per_block)); }
+inline dim3 grid_for(uint64_t items, uint64_t per_block) { return grid_blocks((items + per_block - 1) / per_block); }
 
 template <class T>
 M2<T> to_m2(const rq_c64* m) {

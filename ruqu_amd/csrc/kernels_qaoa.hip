@@ -36,7 +36,7 @@ __device__ __forceinline__ Cx<T> cmul(Cx<T> m, Cx<T> a) {
 
 template <class V, class T>
 __global__ __launch_bounds__(BLOCK) void k_fill_uniform(V* __restrict__ a, uint64_t n, T c) {
-  uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t i = flat_block() * BLOCK + threadIdx.x;
   if (i < n) {
     V v;
     v.x = c;
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(BLOCK) void k_cost_phase(V* __restrict__ a, uint64_
   EdgeF<T>* e = reinterpret_cast<EdgeF<T>*>(smem);
   for (int k = threadIdx.x; k < ne; k += BLOCK) e[k] = edges[k];
   __syncthreads();
-  uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t i = flat_block() * BLOCK + threadIdx.x;
   if (i >= n) return;
   V x = a[i];
   Cx<T> v{x.x, x.y};
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(BLOCK) void k_energy(const V* __restrict__ a, uint6
   for (int k = threadIdx.x; k < ne; k += BLOCK) e[k] = edges[k];
   __syncthreads();
   double acc = 0.0;
-  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK) {
+  for (uint64_t i = flat_block() * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK) {
     V x = a[i];
     double p = (double)x.x * (double)x.x + (double)x.y * (double)x.y;
     double c = 0.0;
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(BLOCK) void k_zmask_batch(const V* __restrict__ a, 
   double acc[ZB];
 #pragma unroll
   for (int t = 0; t < ZB; ++t) acc[t] = 0.0;
-  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK) {
+  for (uint64_t i = flat_block() * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK) {
     V x = a[i];
     double p = (double)x.x * (double)x.x + (double)x.y * (double)x.y;
 #pragma unroll
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(BLOCK) void k_argmax(const V* __restrict__ a, uint6
   __shared__ uint64_t si[BLOCK];
   double bv = -1.0;
   uint64_t bi = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK)
+  for (uint64_t i = flat_block() * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK)
     better(bv, bi, prob_of(a[i]), i);
   sv[threadIdx.x] = bv;
   si[threadIdx.x] = bi;
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(BLOCK) void k_argmax_fold(double* __restrict__ pv, 
   }
 }
 
-inline dim3 grid_for(uint64_t items) { return dim3((unsigned)((items + BLOCK - 1) / BLOCK)); }
+inline dim3 grid_for(uint64_t items) { return grid_blocks((items + BLOCK - 1) / BLOCK); }
 inline unsigned red_blocks(uint64_t n) {
   uint64_t b = (n + BLOCK * 8 - 1) / (BLOCK * 8);
   if (b < 1) b = 1;

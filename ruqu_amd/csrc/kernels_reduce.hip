@@ -46,7 +46,7 @@ __global__ __launch_bounds__(BLOCK) void k_masked_normsq(const V* __restrict__ a
                                                          uint64_t want, double* __restrict__ part) {
   __shared__ double sh[BLOCK / WAVE];
   double acc = 0.0;
-  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK)
+  for (uint64_t i = flat_block() * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK)
     if ((i & mask) == want) acc += nsq(a[i]);
   double r = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = r;
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(BLOCK) void k_pauli(const V* __restrict__ a, uint64
   for (int t = 0; t < nterms; ++t) {
     const PauliDev pd = b.t[t];
     double acc = 0.0;
-    for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK) {
+    for (uint64_t i = flat_block() * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK) {
       V ai = a[i];
       V aj = a[i ^ pd.x];
       // phase = i^ny * s
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(BLOCK) void k_z_all(const V* __restrict__ a, uint64
   double tot = 0.0, p1[40];
 #pragma unroll
   for (int q = 0; q < 40; ++q) p1[q] = 0.0;
-  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK) {
+  for (uint64_t i = flat_block() * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK) {
     double v = nsq(a[i]);
     tot += v;
 #pragma unroll
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner(const V* __restrict__ a, const 
                                                  double* __restrict__ part) {
   __shared__ double sh[BLOCK / WAVE];
   double re = 0.0, im = 0.0;
-  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK) {
+  for (uint64_t i = flat_block() * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK) {
     V x = a[i], y = b[i];
     double xr = x.x, xi = -(double)x.y, yr = y.x, yi = y.y;
     re += xr * yr - xi * yi;
@@ -149,14 +149,14 @@ __global__ __launch_bounds__(BLOCK) void k_inner(const V* __restrict__ a, const 
 // probabilities (state.rs:123-125): f64 re*re+im*im; f32 computed in f32 then
 // widened (mixed_precision.rs:303-305).
 __global__ __launch_bounds__(BLOCK) void k_probs_f64(const double2* __restrict__ a, uint64_t n, double* __restrict__ o) {
-  uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t i = flat_block() * BLOCK + threadIdx.x;
   if (i < n) {
     double2 x = a[i];
     o[i] = x.x * x.x + x.y * x.y;
   }
 }
 __global__ __launch_bounds__(BLOCK) void k_probs_f32(const float2* __restrict__ a, uint64_t n, double* __restrict__ o) {
-  uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t i = flat_block() * BLOCK + threadIdx.x;
   if (i < n) {
     float2 x = a[i];
     o[i] = (double)(x.x * x.x + x.y * x.y);
@@ -168,11 +168,11 @@ __global__ __launch_bounds__(BLOCK) void k_probs_f32(const float2* __restrict__ 
 // probabilities; level k is the sum of the two halves of level k+1.
 template <class V>
 __global__ __launch_bounds__(BLOCK) void k_tree_leaves(const V* __restrict__ a, uint64_t n, double* __restrict__ tree) {
-  uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t i = flat_block() * BLOCK + threadIdx.x;
   if (i < n) tree[n + i] = nsq(a[i]);
 }
 __global__ __launch_bounds__(BLOCK) void k_tree_level(double* __restrict__ tree, uint64_t width) {
-  uint64_t r = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t r = flat_block() * BLOCK + threadIdx.x;
   if (r < width) tree[width + r] = tree[2 * width + r] + tree[3 * width + r];
 }
 
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(BLOCK) void k_tree_level(double* __restrict__ tree,
 __global__ __launch_bounds__(BLOCK) void k_sample(const double* __restrict__ tree, uint32_t nq,
                                                   const double* __restrict__ draws, uint64_t shots,
                                                   uint8_t* __restrict__ bits, unsigned long long* __restrict__ ties) {
-  uint64_t s = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t s = flat_block() * BLOCK + threadIdx.x;
   if (s >= shots) return;
   uint64_t r = 0;
   double kept = 1.0;
@@ -209,7 +209,7 @@ inline unsigned red_blocks(uint64_t n) {
   return (unsigned)b;
 }
 
-inline dim3 grid_for(uint64_t items) { return dim3((unsigned)((items + BLOCK - 1) / BLOCK)); }
+inline dim3 grid_for(uint64_t items) { return grid_blocks((items + BLOCK - 1) / BLOCK); }
 
 // Read `count` doubles from s.red + offset into host `out` (synchronous).
 int read_red(State& s, size_t offset, size_t count, double* out) {

@@ -113,6 +113,56 @@ def test_random_circuit_workload_fused(prec, n):
     assert err < TOL[prec]
 
 
+@pytest.mark.parametrize("n,prec,depth", [(30, "c32", 40), (28, "c64", 20)])
+def test_full_size_fused_matches_exact(n, prec, depth):
+    """The benchmark circuit at full size: the fused schedule against the
+    exact per-gate path (bit-identical to the oracle wherever the oracle runs),
+    through checks that do not depend on the size: norms and the normalised
+    fidelity, computed on the device in FP64."""
+    flush("full-size fused vs exact %s n=%d depth %d" % (prec, n, depth))
+    c = workloads.random_circuit(n, depth, 2)
+    exact = Simulator.run_with_config(c, SimConfig(precision=prec))
+    flush("  exact path done")
+    fused = Simulator.run_with_config(c, SimConfig(precision=prec, fusion=True))
+    nf, ne = fused.state.norm_sq(), exact.state.norm_sq()
+    # |<a|b>|^2 / (|a|^2 |b|^2): the angle between the two states, apart from
+    # the norm drift that the f32 per-gate path accumulates (about 1e-6 here).
+    inf = 1.0 - exact.state.fidelity(fused.state) / (nf * ne)
+    flush("  norms %.12f %.12f  1-F %.3e  passes %d for %d gates" % (nf, ne, inf, fused.metrics.passes,
+                                                                 fused.metrics.gate_count))
+    tol_norm, tol_inf = (1e-4, 1e-9) if prec == "c32" else (1e-12, 1e-12)
+    assert abs(nf - 1.0) < tol_norm and abs(ne - 1.0) < tol_norm
+    assert abs(inf) < tol_inf
+    assert fused.metrics.gate_count == exact.metrics.gate_count
+    fused.state.close()
+    exact.state.close()
+
+
+def test_32_qubit_state():
+    """2^32 amplitudes: every element-wise launch needs more than 2^32 - 1
+    work-items in one grid dimension, so the launches fold into 2-D grids.
+    Bell pair on qubits (0, 31), then a depth-2 random circuit through the
+    exact and the fused path (normalised fidelity on the device)."""
+    n = 32
+    flush("32-qubit c32 state")
+    st = QuantumState.new(n, "c32")
+    st.apply_gate(Gate.H(0))
+    st.apply_gate(Gate.CNOT(0, 31))
+    assert abs(st.probability_of_qubit(31) - 0.5) < 1e-6
+    assert abs(st.expectation_value(PauliString([(0, 3), (31, 3)])) - 1.0) < 1e-6
+    assert abs(st.norm_sq() - 1.0) < 1e-6
+    st.close()
+    c = workloads.random_circuit(n, 2, 5)
+    exact = Simulator.run_with_config(c, SimConfig(precision="c32"))
+    fused = Simulator.run_with_config(c, SimConfig(precision="c32", fusion=True))
+    nf, ne = fused.state.norm_sq(), exact.state.norm_sq()
+    inf = 1.0 - exact.state.fidelity(fused.state) / (nf * ne)
+    flush("  norms %.9f %.9f  1-F %.3e" % (nf, ne, inf))
+    assert abs(nf - 1.0) < 1e-5 and abs(ne - 1.0) < 1e-5 and abs(inf) < 1e-9
+    fused.state.close()
+    exact.state.close()
+
+
 def test_config1_20q_c64():
     """20-qubit random circuit, depth 20, complex-f64: all amplitudes, <Z_i>
     for every qubit and 10^4 measured shots."""
