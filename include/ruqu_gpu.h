@@ -154,8 +154,36 @@ int rq_run_circuit(rq_state* s, const rq_gate* gates, size_t ngates, const doubl
 int rq_apply_circuit(rq_state* s, const rq_gate* gates, size_t ngates, const double* draws, size_t ndraws,
                      rq_measurement* measurements, size_t nout_cap, size_t* nout, int fusion,
                      rq_metrics* metrics);
+/* types.rs:249-253 NoiseModel. */
+typedef struct {
+  double depolarizing_rate;
+  double bit_flip_rate;
+  double phase_flip_rate;
+} rq_noise;
+/* simulator.rs:56-101 with config.noise set: rq_run_circuit, and after every
+ * gate the stochastic channel of apply_noise (simulator.rs:178-221) on each
+ * qubit the gate touches: depolarising (one draw; below the rate a second
+ * draw picks X / Y / Z at 1/3, 2/3), bit flip (one draw), phase flip (one
+ * draw); a channel with rate 0 consumes no draw.  Measurement and noise draws
+ * come from `draws` in the reference's order (the gate's own draw first).
+ * The channel's Paulis are applied like gates but not counted in
+ * metrics->gate_count; *draws_used receives the number of draws consumed. */
+int rq_run_circuit_noisy(rq_state* s, const rq_gate* gates, size_t ngates, const rq_noise* noise,
+                         const double* draws, size_t ndraws, size_t* draws_used, rq_measurement* measurements,
+                         size_t nout_cap, size_t* nout, int fusion, rq_metrics* metrics);
 /* circuit.rs:153-184 QuantumCircuit::depth. */
 uint32_t rq_circuit_depth(uint32_t num_qubits, const rq_gate* gates, size_t ngates);
+
+/* ---- StateCheckpoint (replay.rs:251-294) -------------------------------
+ * The byte format of StateCheckpoint::capture: 2^n amplitudes as
+ * interleaved (re, im) f64 pairs, little-endian, 16 bytes each (complex-f32
+ * states are widened exactly).  rq_state_checkpoint writes 16 * 2^n bytes;
+ * rq_state_from_checkpoint restores (replay.rs:270-287) into a new state of
+ * the given precision (complex-f32 narrows each f64 to nearest);
+ * nbytes must be 16 * 2^n, else InvalidStateVector. */
+size_t rq_state_checkpoint_bytes(const rq_state* s);
+int rq_state_checkpoint(const rq_state* s, void* out);
+int rq_state_from_checkpoint(const void* bytes, size_t nbytes, int precision, rq_state** out);
 
 /* ---- measurement (state.rs:288-352) ------------------------------------ */
 int rq_measure(rq_state* s, uint32_t qubit, double draw, rq_measurement* out);

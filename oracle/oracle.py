@@ -57,6 +57,8 @@ def lib():
             "orc_state_check": (i, [u32, i, i]),
             "orc_apply_gate": (i, [vp, i, u32, vp, vp, sz, C.POINTER(sz), vp, C.POINTER(sz)]),
             "orc_run_circuit": (i, [vp, i, u32, vp, sz, vp, sz, vp, C.POINTER(sz), C.POINTER(sz)]),
+            "orc_run_circuit_noisy": (i, [vp, i, u32, vp, sz, d, d, d, vp, sz, vp, C.POINTER(sz), C.POINTER(sz),
+                                          C.POINTER(sz)]),
             "orc_measure": (i, [vp, i, u32, u32, d, C.POINTER(i), C.POINTER(d)]),
             "orc_measure_all": (i, [vp, i, u32, vp, vp, vp]),
             "orc_run_shots": (i, [i, u32, vp, sz, sz, vp, sz, vp]),
@@ -123,6 +125,23 @@ def run_circuit(n, gates, f32=False, draws=None):
     if rc:
         raise RuntimeError("oracle run_circuit failed with status %d" % rc)
     return amps, outs[: nout.value].copy(), cnt.value
+
+
+def run_circuit_noisy(n, gates, depol, bit_flip, phase_flip, draws, f32=False):
+    """Simulator::run_with_config with a NoiseModel (simulator.rs:56-101,
+    apply_noise :178-221) from |0..0>; returns (amplitudes, outcomes,
+    gate_count, draws_used)."""
+    g = gate_array(gates)
+    amps = np.zeros(1 << n, dtype=np.complex64 if f32 else np.complex128)
+    d = np.ascontiguousarray(draws, dtype=np.float64)
+    outs = np.zeros(max(len(g), 1), dtype=OUTCOME_DTYPE)
+    nout, cnt, used = C.c_size_t(0), C.c_size_t(0), C.c_size_t(0)
+    rc = lib().orc_run_circuit_noisy(_p(amps), 1 if f32 else 0, n, _p(g), len(g), float(depol), float(bit_flip),
+                                     float(phase_flip), _p(d), len(d), _p(outs), C.byref(nout), C.byref(cnt),
+                                     C.byref(used))
+    if rc:
+        raise RuntimeError("oracle run_circuit_noisy failed with status %d" % rc)
+    return amps, outs[: nout.value].copy(), cnt.value, used.value
 
 
 def apply_gate(amps, n, gate, draw=0.0):

@@ -457,6 +457,65 @@ int orc_run_circuit(void* amps, int f32, uint32_t nq, const orc_gate* gates, siz
   return E_OK;
 }
 
+// simulator.rs:56-101 with a noise model: after every gate, apply_noise
+// (simulator.rs:178-221) for each qubit the gate touches (Gate::qubits,
+// gate.rs:40-64): depolarising (one draw; below the rate a second draw picks
+// X / Y / Z at 1/3, 2/3), then bit flip (one draw, X), then phase flip (one
+// draw, Z); a channel with rate 0 consumes nothing.  Measurement and noise
+// draws come from one stream in that order.  *draws_used = draws consumed.
+int orc_run_circuit_noisy(void* amps, int f32, uint32_t nq, const orc_gate* gates, size_t ngates,
+                          double depol, double bit_flip, double phase_flip, const double* draws,
+                          size_t ndraws, orc_outcome* outs, size_t* nouts, size_t* gate_count,
+                          size_t* draws_used) {
+  size_t n = (size_t)1 << nq;
+  if (f32) {
+    C32* a = (C32*)amps;
+    for (size_t i = 0; i < n; ++i) a[i] = C32{0.0f, 0.0f};
+    a[0] = C32{1.0f, 0.0f};
+  } else {
+    C64* a = (C64*)amps;
+    for (size_t i = 0; i < n; ++i) a[i] = C64{0.0, 0.0};
+    a[0] = C64{1.0, 0.0};
+  }
+  size_t cursor = 0, cnt = 0;
+  *nouts = 0;
+  auto pauli = [&](int kind, uint32_t q) {
+    orc_gate p{};
+    p.kind = kind;
+    p.q0 = q;
+    return apply_gate_any(amps, f32, nq, p, nullptr, 0, nullptr, nullptr, nullptr);
+  };
+  for (size_t k = 0; k < ngates; ++k) {
+    int rc = apply_gate_any(amps, f32, nq, gates[k], draws, ndraws, &cursor, outs, nouts);
+    if (rc) return rc;
+    if (!is_non_unitary(gates[k].kind)) ++cnt;
+    uint32_t qs[2];
+    const int nqs = gate_qubits(gates[k], qs);
+    for (int t = 0; t < nqs; ++t) {
+      if (depol > 0.0) {
+        if (cursor >= ndraws) return E_CIRCUIT;
+        if (draws[cursor++] < depol) {
+          if (cursor >= ndraws) return E_CIRCUIT;
+          const double choice = draws[cursor++];
+          rc = pauli(choice < 1.0 / 3.0 ? G_X : (choice < 2.0 / 3.0 ? G_Y : G_Z), qs[t]);
+          if (rc) return rc;
+        }
+      }
+      if (bit_flip > 0.0) {
+        if (cursor >= ndraws) return E_CIRCUIT;
+        if (draws[cursor++] < bit_flip && (rc = pauli(G_X, qs[t]))) return rc;
+      }
+      if (phase_flip > 0.0) {
+        if (cursor >= ndraws) return E_CIRCUIT;
+        if (draws[cursor++] < phase_flip && (rc = pauli(G_Z, qs[t]))) return rc;
+      }
+    }
+  }
+  if (gate_count) *gate_count = cnt;
+  if (draws_used) *draws_used = cursor;
+  return E_OK;
+}
+
 // state.rs:288-326 / mixed_precision.rs:473-515
 int orc_measure(void* amps, int f32, uint32_t nq, uint32_t q, double draw, int* result, double* prob) {
   if (q >= nq) return E_INVALID_QUBIT;

@@ -25,11 +25,12 @@ from .state import (
     SimulationMetrics,
     SimulationResult,
     Simulator,
+    StateCheckpoint,
 )
 
 __all__ = [
     "Gate", "GateKind", "Hamiltonian", "PauliOp", "PauliString", "QuantumCircuit", "ArgumentError", "CircuitError",
     "DeviceError", "InvalidQubitIndex", "InvalidStateVector", "MeasurementOutcome", "MemoryAllocationFailed",
     "NoiseModel", "QuantumError", "QuantumState", "QuantumStateF32", "QubitLimitExceeded", "ShotResult", "SimConfig",
-    "SimulationMetrics", "SimulationResult", "Simulator",
+    "SimulationMetrics", "SimulationResult", "Simulator", "StateCheckpoint",
 ]

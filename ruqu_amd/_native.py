@@ -38,6 +38,10 @@ class Metrics(C.Structure):
     ]
 
 
+class Noise(C.Structure):  # rq_noise (types.rs:249-253)
+    _fields_ = [("depolarizing_rate", C.c_double), ("bit_flip_rate", C.c_double), ("phase_flip_rate", C.c_double)]
+
+
 class Measurement(C.Structure):
     _fields_ = [("qubit", C.c_uint32), ("result", C.c_int32), ("probability", C.c_double)]
 
@@ -71,7 +75,12 @@ SIGNATURES = {
     "rq_apply_2q": (_i, [_vp, _u32, _u32, _vp]),
     "rq_run_circuit": (_i, [_vp, _vp, _sz, _vp, _sz, _vp, _sz, C.POINTER(_sz), _i, C.POINTER(Metrics)]),
     "rq_apply_circuit": (_i, [_vp, _vp, _sz, _vp, _sz, _vp, _sz, C.POINTER(_sz), _i, C.POINTER(Metrics)]),
+    "rq_run_circuit_noisy": (_i, [_vp, _vp, _sz, C.POINTER(Noise), _vp, _sz, C.POINTER(_sz), _vp, _sz,
+                                 C.POINTER(_sz), _i, C.POINTER(Metrics)]),
     "rq_circuit_depth": (_u32, [_u32, _vp, _sz]),
+    "rq_state_checkpoint_bytes": (_sz, [_vp]),
+    "rq_state_checkpoint": (_i, [_vp, _vp]),
+    "rq_state_from_checkpoint": (_i, [_vp, _sz, _i, C.POINTER(_vp)]),
     "rq_measure": (_i, [_vp, _u32, _d, C.POINTER(Measurement)]),
     "rq_measure_all": (_i, [_vp, _vp, _vp]),
     "rq_reset_qubit": (_i, [_vp, _u32, _d]),

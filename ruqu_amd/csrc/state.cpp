@@ -4,6 +4,7 @@
 // Validation and error texts follow error.rs:8-23 and the checks in
 // state.rs:31-96,158-204,441-449; every failure returns a status and sets the
 // thread-local message, nothing aborts.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -362,6 +363,65 @@ int rq_state_from_amplitudes(const void* host, size_t len, uint32_t nq, int prec
   return RQ_OK;
 }
 
+size_t rq_state_checkpoint_bytes(const rq_state* h) {  // replay.rs:290-293
+  const State* s = (const State*)h;
+  return s ? s->n * 16 : 0;
+}
+
+// replay.rs:257-268: interleaved little-endian f64 (re, im).  A complex-f64
+// state's device buffer is exactly that; complex-f32 amplitudes are copied
+// out in chunks and widened (exact).
+int rq_state_checkpoint(const rq_state* h, void* out) {
+  const State* s = (const State*)h;
+  if (!s || !out) return fail(RQ_ERR_ARGUMENT, "null pointer");
+  RQ_HIP(hipStreamSynchronize(s->stream));
+  if (s->prec == RQ_C64) {
+    RQ_HIP(hipMemcpy(out, s->amps, s->n * 16, hipMemcpyDeviceToHost));
+    return RQ_OK;
+  }
+  const size_t chunk = size_t(1) << 22;  // amplitudes per staging copy
+  std::vector<float> stage(2 * std::min(chunk, s->n));
+  double* o = (double*)out;
+  for (size_t i0 = 0; i0 < s->n; i0 += chunk) {
+    const size_t m = std::min(chunk, s->n - i0);
+    RQ_HIP(hipMemcpy(stage.data(), (const float*)s->amps + 2 * i0, m * 8, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < 2 * m; ++k) o[2 * i0 + k] = (double)stage[k];
+  }
+  return RQ_OK;
+}
+
+// replay.rs:270-287 then QuantumState::from_amplitudes (state.rs:77-96).
+int rq_state_from_checkpoint(const void* bytes, size_t nbytes, int precision, rq_state** out) {
+  if (!out || (!bytes && nbytes)) return fail(RQ_ERR_ARGUMENT, "null pointer");
+  if (precision != RQ_C64 && precision != RQ_C32) return fail(RQ_ERR_ARGUMENT, "unknown precision");
+  const size_t count = nbytes / 16;
+  if (nbytes % 16 || count == 0 || (count & (count - 1)))
+    return fail(RQ_ERR_INVALID_STATE, "invalid state vector: checkpoint of " + std::to_string(nbytes) +
+                                          " bytes is not 16 * 2^n");
+  uint32_t nq = 0;
+  while ((size_t(1) << nq) < count) ++nq;
+  if (precision == RQ_C64) return rq_state_from_amplitudes(bytes, count, nq, RQ_C64, out);
+  uint32_t maxq = rq_max_qubits(precision);
+  if (nq > maxq) return limit_error(nq, (int)maxq);
+  State* s = nullptr;
+  int rc = alloc_state(nq, precision, &s);
+  if (rc) return rc;
+  const size_t chunk = size_t(1) << 22;
+  std::vector<float> stage(2 * std::min(chunk, count));
+  const double* in = (const double*)bytes;
+  for (size_t i0 = 0; i0 < count; i0 += chunk) {
+    const size_t m = std::min(chunk, count - i0);
+    for (size_t k = 0; k < 2 * m; ++k) stage[k] = (float)in[2 * i0 + k];
+    hipError_t e = hipMemcpy((float*)s->amps + 2 * i0, stage.data(), m * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      rq_state_destroy((rq_state*)s);
+      return hip_fail(e, "from_checkpoint");
+    }
+  }
+  *out = (rq_state*)s;
+  return RQ_OK;
+}
+
 void rq_state_destroy(rq_state* h) {
   State* s = (State*)h;
   if (!s) return;
@@ -460,6 +520,71 @@ int rq_run_circuit(rq_state* h, const rq_gate* gates, size_t ngates, const doubl
   State* s = (State*)h;
   if (!s) return fail(RQ_ERR_ARGUMENT, "null state");
   return run_gates(*s, gates, ngates, draws, ndraws, meas, cap, nout, fusion, metrics, true);
+}
+
+// simulator.rs:56-101 with a noise model.  The channel's choices depend on
+// the draws only, never on the state, so the circuit is expanded on the host
+// into gates plus the Paulis the channel applies (apply_noise,
+// simulator.rs:178-221, per qubit of Gate::qubits, gate.rs:40-64) and run
+// through the ordinary path; the Measure / Reset draws are taken from the
+// same stream in the reference's order (the gate's draw before its noise).
+int rq_run_circuit_noisy(rq_state* h, const rq_gate* gates, size_t ngates, const rq_noise* noise,
+                         const double* draws, size_t ndraws, size_t* draws_used, rq_measurement* meas,
+                         size_t cap, size_t* nout, int fusion, rq_metrics* metrics) {
+  State* s = (State*)h;
+  if (!s) return fail(RQ_ERR_ARGUMENT, "null state");
+  if ((ngates && !gates) || (ndraws && !draws)) return fail(RQ_ERR_ARGUMENT, "null pointer");
+  const double dp = noise ? noise->depolarizing_rate : 0.0, bf = noise ? noise->bit_flip_rate : 0.0,
+               pf = noise ? noise->phase_flip_rate : 0.0;
+  std::vector<rq_gate> ex;
+  std::vector<double> md;
+  ex.reserve(ngates + ngates / 4 + 8);
+  size_t cur = 0;
+  uint64_t inserted = 0;
+  const std::string exhausted = "noise channel needs a uniform draw but the draw list is exhausted";
+  auto pauli = [&](int kind, uint32_t q) {
+    rq_gate p{};
+    p.kind = kind;
+    p.q0 = q;
+    ex.push_back(p);
+    ++inserted;
+  };
+  for (size_t k = 0; k < ngates; ++k) {
+    const rq_gate& g = gates[k];
+    if (g.kind == RQ_GATE_MEASURE || g.kind == RQ_GATE_RESET) {
+      if (cur >= ndraws) return fail(RQ_ERR_CIRCUIT, "measurement needs a uniform draw but the draw list is exhausted");
+      md.push_back(draws[cur++]);
+    }
+    ex.push_back(g);
+    const uint32_t qs[2] = {g.q0, g.q1};
+    const int nqs = g.kind == RQ_GATE_BARRIER ? 0 : (is_2q(g.kind) ? 2 : 1);
+    for (int t = 0; t < nqs; ++t) {
+      if (dp > 0.0) {
+        if (cur >= ndraws) return fail(RQ_ERR_CIRCUIT, exhausted);
+        if (draws[cur++] < dp) {
+          if (cur >= ndraws) return fail(RQ_ERR_CIRCUIT, exhausted);
+          const double choice = draws[cur++];
+          pauli(choice < 1.0 / 3.0 ? RQ_GATE_X : (choice < 2.0 / 3.0 ? RQ_GATE_Y : RQ_GATE_Z), qs[t]);
+        }
+      }
+      if (bf > 0.0) {
+        if (cur >= ndraws) return fail(RQ_ERR_CIRCUIT, exhausted);
+        if (draws[cur++] < bf) pauli(RQ_GATE_X, qs[t]);
+      }
+      if (pf > 0.0) {
+        if (cur >= ndraws) return fail(RQ_ERR_CIRCUIT, exhausted);
+        if (draws[cur++] < pf) pauli(RQ_GATE_Z, qs[t]);
+      }
+    }
+  }
+  if (draws_used) *draws_used = cur;
+  int rc = run_gates(*s, ex.data(), ex.size(), md.data(), md.size(), meas, cap, nout, fusion, metrics, true);
+  if (rc == RQ_OK && metrics) {  // the channel's Paulis are not circuit gates (simulator.rs:73-75)
+    metrics->gate_count -= inserted;
+    const double secs = metrics->execution_time_ns * 1e-9;
+    metrics->gates_per_second = secs > 0 ? metrics->gate_count / secs : 0.0;
+  }
+  return rc;
 }
 
 int rq_apply_circuit(rq_state* h, const rq_gate* gates, size_t ngates, const double* draws, size_t ndraws,

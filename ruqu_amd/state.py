@@ -17,7 +17,7 @@ import time
 import numpy as np
 
 from . import _native
-from ._native import MEAS_DTYPE, Measurement, Metrics, lib, ptr
+from ._native import MEAS_DTYPE, Measurement, Metrics, Noise, lib, ptr
 from .gates import GateKind, QuantumCircuit
 
 C64, C32 = 0, 1
@@ -134,6 +134,24 @@ class QuantumState:
         h = C.c_void_p()
         check(lib().rq_state_from_amplitudes(ptr(a), a.size, int(num_qubits), p, C.byref(h)))
         return cls(h.value, p)
+
+    @classmethod
+    def from_checkpoint(cls, checkpoint, precision="c64"):
+        """A new device state from StateCheckpoint bytes (replay.rs:270-287,
+        then from_amplitudes); 'c32' narrows each f64 to nearest."""
+        data = checkpoint.data if isinstance(checkpoint, StateCheckpoint) else bytes(checkpoint)
+        p = _as_prec(precision)
+        buf = np.frombuffer(data, dtype=np.uint8)
+        h = C.c_void_p()
+        check(lib().rq_state_from_checkpoint(ptr(buf) if buf.size else None, buf.size, p, C.byref(h)))
+        return cls(h.value, p)
+
+    def checkpoint(self):
+        """StateCheckpoint::capture of this state's amplitudes (replay.rs:257-268):
+        little-endian f64 (re, im) pairs, complex-f32 widened exactly."""
+        out = np.empty(int(lib().rq_state_checkpoint_bytes(self._h)), dtype=np.uint8)
+        check(lib().rq_state_checkpoint(self._h, ptr(out)))
+        return StateCheckpoint(out.tobytes(), out.size // 16)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -290,6 +308,28 @@ class QuantumState:
 QuantumStateF32 = QuantumState  # precision='c32' selects the f32 twin
 
 
+class StateCheckpoint:
+    """replay.rs:251-294: a serialised snapshot of a state vector, amplitudes
+    as interleaved little-endian f64 (re, im) pairs, 16 bytes each."""
+
+    def __init__(self, data, num_amplitudes):
+        self.data = bytes(data)
+        self.num_amplitudes = int(num_amplitudes)
+
+    @classmethod
+    def capture(cls, amplitudes):  # replay.rs:257-268
+        a = np.ascontiguousarray(amplitudes, dtype=np.complex128)
+        return cls(a.astype("<c16").tobytes(), a.size)
+
+    def restore(self):  # replay.rs:270-287
+        if len(self.data) < 16 * self.num_amplitudes:
+            raise ValueError("checkpoint data corrupted")
+        return np.frombuffer(self.data, dtype="<c16", count=self.num_amplitudes).astype(np.complex128)
+
+    def size_bytes(self):  # replay.rs:290-293
+        return len(self.data)
+
+
 class NoiseModel:
     """types.rs:249-253."""
 
@@ -346,9 +386,10 @@ class Simulator:
 
     @staticmethod
     def run_with_config(circuit, config, state=None):
-        if config.noise is not None and any(
-                getattr(config.noise, f) > 0 for f in ("depolarizing_rate", "bit_flip_rate", "phase_flip_rate")):
-            raise NotImplementedError("stochastic noise trajectories are not implemented on the device path")
+        noise = config.noise
+        if noise is not None and any(
+                getattr(noise, f) > 0 for f in ("depolarizing_rate", "bit_flip_rate", "phase_flip_rate")):
+            return Simulator._run_noisy(circuit, config, state)
         arr = circuit.to_array()
         p = _as_prec(config.precision)
         if state is None:
@@ -368,6 +409,49 @@ class Simulator:
                                    C.byref(nout), _fusion_mode(config.fusion), C.byref(met)))
         outs = [MeasurementOutcome(r["qubit"], r["result"], r["probability"]) for r in meas[: nout.value]]
         return SimulationResult(state, outs, SimulationMetrics(met))
+
+    @staticmethod
+    def noise_draw_bound(circuit, noise):
+        """Most uniform draws a run with `noise` can consume: one per
+        Measure/Reset plus, per qubit a gate touches, two for depolarising
+        (when its rate > 0), one for bit flip and one for phase flip."""
+        arr = circuit.to_array()
+        k = arr["kind"]
+        two = (k >= GateKind.CNOT) & (k <= GateKind.Rzz)
+        touched = int(np.count_nonzero(k != GateKind.Barrier)) + int(np.count_nonzero(two))
+        per = (2 if noise.depolarizing_rate > 0 else 0) + (noise.bit_flip_rate > 0) + (noise.phase_flip_rate > 0)
+        return _count_draws(arr) + touched * per
+
+    @staticmethod
+    def _run_noisy(circuit, config, state=None):
+        """simulator.rs:56-101 with config.noise (apply_noise, :178-221):
+        one trajectory; config.draws (if given) is the stream of uniforms
+        the reference's StdRng would supply, measurement and noise draws
+        interleaved in the reference's order."""
+        arr = circuit.to_array()
+        p = _as_prec(config.precision)
+        if state is None:
+            if config.seed is not None:
+                state = QuantumState.new_with_seed(circuit.num_qubits(), config.seed, p)
+            else:
+                state = QuantumState.new(circuit.num_qubits(), p)
+        nz = config.noise
+        if config.draws is not None:
+            draws = np.ascontiguousarray(config.draws, dtype=np.float64)
+        else:
+            draws = state._rng.random(Simulator.noise_draw_bound(circuit, nz))
+        nm = _count_draws(arr)
+        meas = np.zeros(max(nm, 1), dtype=MEAS_DTYPE)
+        nout, used = C.c_size_t(0), C.c_size_t(0)
+        met = Metrics()
+        cn = Noise(float(nz.depolarizing_rate), float(nz.bit_flip_rate), float(nz.phase_flip_rate))
+        check(lib().rq_run_circuit_noisy(state.handle, ptr(arr), len(arr), C.byref(cn), ptr(draws), len(draws),
+                                         C.byref(used), ptr(meas), len(meas), C.byref(nout),
+                                         _fusion_mode(config.fusion), C.byref(met)))
+        outs = [MeasurementOutcome(r["qubit"], r["result"], r["probability"]) for r in meas[: nout.value]]
+        res = SimulationResult(state, outs, SimulationMetrics(met))
+        res.draws_used = used.value
+        return res
 
     @staticmethod
     def run_shots(circuit, shots, seed=None, precision="c64", fusion=False, draws=None):
