@@ -185,6 +185,33 @@ size_t rq_state_checkpoint_bytes(const rq_state* s);
 int rq_state_checkpoint(const rq_state* s, void* out);
 int rq_state_from_checkpoint(const void* bytes, size_t nbytes, int precision, rq_state** out);
 
+/* ---- Grover search (crates/ruqu-algorithms/src/grover.rs) ---------------
+ * rq_negate_amplitudes: the oracle step grover.rs:130-139 performs through
+ * amplitudes_mut: a[i] = -a[i] for every listed index (an index listed twice
+ * is negated twice); indices >= 2^n give InvalidStateVector.
+ * rq_negate_amplitudes_from: a[i] = -a[i] for i >= start (grover.rs:146-155
+ * with start = 1). */
+int rq_negate_amplitudes(rq_state* s, const uint64_t* indices, size_t count);
+int rq_negate_amplitudes_from(rq_state* s, uint64_t start);
+/* grover.rs:69-74 optimal_iterations (max(round(pi/4/theta - 1/2), 1),
+ * theta = asin(sqrt(M/N)); saturates like Rust's `as u32`). */
+uint32_t rq_grover_optimal_iterations(uint32_t num_qubits, size_t num_targets);
+/* grover.rs:42-55 GroverResult without the state. */
+typedef struct {
+  uint64_t measured_state;
+  int32_t target_found;
+  uint32_t num_iterations;
+  double success_probability;
+} rq_grover_result;
+/* grover.rs:98-183 run_grover: iterations < 0 selects optimal_iterations;
+ * draws[num_qubits] are the uniforms of the final measurement of qubits
+ * 0..n-1 (grover.rs:187-200); a target >= 2^n is an argument error (an
+ * assert in the reference); an empty target list with iterations < 0 is an
+ * argument error (the reference's count is unbounded).  state_out (optional)
+ * receives the final state (the caller destroys it). */
+int rq_grover_run(uint32_t num_qubits, const uint64_t* targets, size_t num_targets, int64_t iterations,
+                  int precision, int with_seed, const double* draws, rq_grover_result* out, rq_state** state_out);
+
 /* ---- measurement (state.rs:288-352) ------------------------------------ */
 int rq_measure(rq_state* s, uint32_t qubit, double draw, rq_measurement* out);
 /* state.rs:329-335: draws[q] is consumed by qubit q, qubit 0 first. */

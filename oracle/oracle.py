@@ -60,6 +60,9 @@ def lib():
             "orc_run_circuit_noisy": (i, [vp, i, u32, vp, sz, d, d, d, vp, sz, vp, C.POINTER(sz), C.POINTER(sz),
                                           C.POINTER(sz)]),
             "orc_measure": (i, [vp, i, u32, u32, d, C.POINTER(i), C.POINTER(d)]),
+            "orc_grover_optimal_iterations": (u32, [u32, sz]),
+            "orc_run_grover": (i, [u32, vp, sz, C.c_int64, vp, vp, C.POINTER(C.c_uint64), C.POINTER(i),
+                                   C.POINTER(d), C.POINTER(u32)]),
             "orc_measure_all": (i, [vp, i, u32, vp, vp, vp]),
             "orc_run_shots": (i, [i, u32, vp, sz, sz, vp, sz, vp]),
             "orc_probabilities": (None, [vp, i, u32, vp]),
@@ -142,6 +145,25 @@ def run_circuit_noisy(n, gates, depol, bit_flip, phase_flip, draws, f32=False):
     if rc:
         raise RuntimeError("oracle run_circuit_noisy failed with status %d" % rc)
     return amps, outs[: nout.value].copy(), cnt.value, used.value
+
+
+def grover_optimal_iterations(n, m):
+    return int(lib().orc_grover_optimal_iterations(n, m))
+
+
+def run_grover(n, targets, iterations, draws):
+    """grover.rs:98-183 run_grover (f64); iterations=None selects the optimal
+    count.  Returns (final amplitudes, measured_state, target_found,
+    success_probability, num_iterations)."""
+    t = np.ascontiguousarray(targets, dtype=np.uint64)
+    dr = np.ascontiguousarray(draws, dtype=np.float64)
+    amps = np.zeros(1 << n, dtype=np.complex128)
+    m, f, p, it = C.c_uint64(0), C.c_int(0), C.c_double(0), C.c_uint32(0)
+    rc = lib().orc_run_grover(n, _p(t) if t.size else None, t.size, -1 if iterations is None else int(iterations),
+                              _p(dr) if dr.size else None, _p(amps), C.byref(m), C.byref(f), C.byref(p), C.byref(it))
+    if rc:
+        raise RuntimeError("oracle run_grover failed with status %d" % rc)
+    return amps, m.value, bool(f.value), p.value, it.value
 
 
 def apply_gate(amps, n, gate, draw=0.0):

@@ -516,6 +516,67 @@ int orc_run_circuit_noisy(void* amps, int f32, uint32_t nq, const orc_gate* gate
   return E_OK;
 }
 
+// grover.rs:69-74 optimal_iterations: max(round(pi/4 / theta - 0.5), 1) with
+// theta = asin(sqrt(M / N)); Rust's f64::max ignores NaN (fmax), `as u32`
+// saturates.
+uint32_t orc_grover_optimal_iterations(uint32_t nq, size_t m) {
+  const double n = (double)((size_t)1 << nq);
+  const double theta = std::asin(std::sqrt((double)m / n));
+  const double k = std::fmax(std::round(0.78539816339744830962 / theta - 0.5), 1.0);
+  if (!(k < 4294967295.0)) return 4294967295u;
+  return (uint32_t)k;
+}
+
+// grover.rs:98-183 run_grover (QuantumState, f64): H on every qubit; per
+// iteration negate each listed target (:130-139), H on every qubit, negate
+// every amplitude but index 0 (:146-155), H on every qubit; success
+// probability = sum of probabilities()[t] over the list in order (:161-162);
+// measure qubits 0..n-1 in turn with draws[q] (:187-200).  iterations < 0
+// selects optimal_iterations.  amps_out receives the final (collapsed) state.
+int orc_run_grover(uint32_t nq, const uint64_t* targets, size_t nt, int64_t iterations, const double* draws,
+                   double* amps_out, uint64_t* measured, int* found, double* psucc, uint32_t* iters) {
+  const size_t n = (size_t)1 << nq;
+  for (size_t k = 0; k < nt; ++k)
+    if (targets[k] >= n) return E_CIRCUIT;  // an assert (panic) in the reference
+  const uint32_t it_count = iterations < 0 ? orc_grover_optimal_iterations(nq, nt) : (uint32_t)iterations;
+  std::vector<C64> a(n, C64{0.0, 0.0});
+  a[0] = C64{1.0, 0.0};
+  orc_gate hg{};
+  hg.kind = G_H;
+  C64 h[2][2];
+  matrix_1q(hg, h);
+  auto hall = [&]() {
+    for (uint32_t q = 0; q < nq; ++q) apply_1q(a.data(), n, q, h);
+  };
+  hall();
+  for (uint32_t it = 0; it < it_count; ++it) {
+    for (size_t k = 0; k < nt; ++k) a[targets[k]] = C64{-a[targets[k]].re, -a[targets[k]].im};
+    hall();
+    for (size_t i = 1; i < n; ++i) a[i] = C64{-a[i].re, -a[i].im};
+    hall();
+  }
+  double ps = 0.0;
+  for (size_t k = 0; k < nt; ++k) {
+    const C64 x = a[targets[k]];
+    ps += x.re * x.re + x.im * x.im;
+  }
+  uint64_t m = 0;
+  for (uint32_t q = 0; q < nq; ++q) {
+    int res;
+    double p;
+    measure64(a.data(), n, q, draws[q], &res, &p);
+    if (res) m |= (uint64_t)1 << q;
+  }
+  *measured = m;
+  *found = 0;
+  for (size_t k = 0; k < nt; ++k)
+    if (targets[k] == m) *found = 1;
+  *psucc = ps;
+  *iters = it_count;
+  if (amps_out) memcpy(amps_out, a.data(), n * sizeof(C64));
+  return E_OK;
+}
+
 // state.rs:288-326 / mixed_precision.rs:473-515
 int orc_measure(void* amps, int f32, uint32_t nq, uint32_t q, double draw, int* result, double* prob) {
   if (q >= nq) return E_INVALID_QUBIT;

@@ -79,6 +79,8 @@ void launch_swap(State& s, uint32_t q1, uint32_t q2);
 void launch_diag_2q(State& s, uint32_t q1, uint32_t q2, const rq_c64* d4);
 void launch_collapse(State& s, uint32_t q, int result, double nf64);
 void launch_scale(State& s, double f);
+void launch_negate_from(State& s, uint64_t start);                       // a[i] = -a[i], i >= start
+void launch_negate_listed(State& s, const uint64_t* d_idx, uint64_t count);  // distinct device indices
 void launch_copy(State& dst, const State& src);
 void launch_plain_copy(void* dst, const void* src, size_t bytes, hipStream_t st);
 

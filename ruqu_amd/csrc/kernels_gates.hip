@@ -281,6 +281,30 @@ __global__ __launch_bounds__(BLOCK) void k_scale(V* __restrict__ a, T f, uint64_
   a[i] = x;
 }
 
+// ---- sign flips (grover.rs:118-129 oracle, :136-143 diffusion) ------------
+// Negates a[i] for i in [start, n): a sign flip of both parts, exact.
+template <class V>
+__global__ __launch_bounds__(BLOCK) void k_negate_from(V* __restrict__ a, uint64_t start, uint64_t n) {
+  uint64_t i = start + flat_block() * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  V x = a[i];
+  x.x = -x.x;
+  x.y = -x.y;
+  a[i] = x;
+}
+// Negates a[idx[k]] for k < count; the indices are distinct (the host folds
+// repeated indices: an index listed twice is negated twice, i.e. not at all).
+template <class V>
+__global__ __launch_bounds__(BLOCK) void k_negate_listed(V* __restrict__ a, const uint64_t* __restrict__ idx,
+                                                         uint64_t count) {
+  uint64_t k = flat_block() * BLOCK + threadIdx.x;
+  if (k >= count) return;
+  V x = a[idx[k]];
+  x.x = -x.x;
+  x.y = -x.y;
+  a[idx[k]] = x;
+}
+
 // ---- plain 16 B copy (bandwidth reference and state copies) ----------------
 template <int U, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_copy16(float4* __restrict__ dst, const float4* __restrict__ src,
@@ -434,6 +458,27 @@ void launch_scale(State& s, double f) {
   else
     hipLaunchKernelGGL((k_scale<float, float2>), grid_for(s.n, BLOCK), dim3(BLOCK), 0, s.stream, (float2*)s.amps,
                        (float)f, s.n);
+}
+
+void launch_negate_from(State& s, uint64_t start) {
+  if (start >= s.n) return;
+  const uint64_t m = s.n - start;
+  if (s.prec == RQ_C64)
+    hipLaunchKernelGGL(k_negate_from<double2>, grid_for(m, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, start,
+                       (uint64_t)s.n);
+  else
+    hipLaunchKernelGGL(k_negate_from<float2>, grid_for(m, BLOCK), dim3(BLOCK), 0, s.stream, (float2*)s.amps, start,
+                       (uint64_t)s.n);
+}
+
+void launch_negate_listed(State& s, const uint64_t* d_idx, uint64_t count) {
+  if (!count) return;
+  if (s.prec == RQ_C64)
+    hipLaunchKernelGGL(k_negate_listed<double2>, grid_for(count, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps,
+                       d_idx, count);
+  else
+    hipLaunchKernelGGL(k_negate_listed<float2>, grid_for(count, BLOCK), dim3(BLOCK), 0, s.stream, (float2*)s.amps,
+                       d_idx, count);
 }
 
 void launch_plain_copy(void* dst, const void* src, size_t bytes, hipStream_t st) {
