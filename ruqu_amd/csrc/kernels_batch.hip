@@ -33,6 +33,7 @@ constexpr int B = BATCH_THREADS;
 constexpr int WAVE = 64;
 constexpr int LQ = BATCH_LDS_QUBITS;
 constexpr double TWO_PI = 6.283185307179586476925286766559;
+constexpr double INV_TWO_PI = 0.15915494309189533576888376337251;
 
 __device__ __forceinline__ uint32_t ins0(uint32_t p, uint32_t q) {
   uint32_t lo = p & ((1u << q) - 1u);
@@ -116,11 +117,15 @@ __device__ __forceinline__ double cut_of(uint64_t i, const BEdge* __restrict__ e
   return c;
 }
 
+// e^{i phi} a, phi = gamma (2 C - W).  The angle is reduced in FP64 to turns,
+// r = phi / 2pi - nearest integer (|r| <= 1/2), and evaluated by the
+// hardware sine / cosine, which take turns (v_sin_f32, v_cos_f32: a few
+// instructions instead of a full-range sincosf and an FP64 division).
 __device__ __forceinline__ float2 phase_mul(float2 a, double gamma, double cut, double wsum) {
-  double phi = gamma * (2.0 * cut - wsum);
-  phi -= TWO_PI * rint(phi / TWO_PI);
-  float sn, cs;
-  sincosf((float)phi, &sn, &cs);
+  double r = gamma * (2.0 * cut - wsum) * INV_TWO_PI;
+  r -= rint(r);
+  const float rf = (float)r;
+  const float sn = __builtin_amdgcn_sinf(rf), cs = __builtin_amdgcn_cosf(rf);
   return make_float2(cs * a.x - sn * a.y, cs * a.y + sn * a.x);
 }
 

@@ -65,6 +65,9 @@ inline int __builtin_amdgcn_readfirstlane(int v) {
   emu::wait(emu::wave_bar[w]);
   return r;
 }
+// The hardware sine / cosine take the angle in turns.
+inline float __builtin_amdgcn_sinf(float r) { return (float)std::sin(6.283185307179586476925 * (double)r); }
+inline float __builtin_amdgcn_cosf(float r) { return (float)std::cos(6.283185307179586476925 * (double)r); }
 inline int max(int a, int b) { return a > b ? a : b; }
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int __popc(uint32_t x) { return __builtin_popcount(x); }
