@@ -20,11 +20,17 @@
 // |a_i|^2 (ties to the larger index, qaoa.rs:249-255).
 // VQE: Ry(theta_{l,q}) on every qubit, then CZ(q, q+1) for q < n-1, per
 // layer; energy sum_k c_k <P_k> (state.rs:362-408 semantics per term).
+//
+// Arithmetic: FP contraction is allowed (the batch has no bit-exact reference
+// loop; results are held to the FP64 oracle within 1e-5 of the energy scale),
+// so a rotation's outputs are one multiply and one FMA each.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 
 #include "batch.h"
+
+#pragma clang fp contract(fast)
 
 namespace rq {
 namespace {
