@@ -361,6 +361,59 @@ __device__ __forceinline__ void wide_transpose(float2 (&v)[1 << (TQ - 8)], int f
   __syncthreads();
 }
 
+// A copy of x in new registers (see rot_step).
+__device__ __forceinline__ float fresh(float x) {
+  float r;
+  asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
+// One bit of a window rotation, x -> y (distinct register arrays): Rx or Ry
+// with (c, s) = (cos, sin) of half the angle on register bit JB, or, when the
+// bit is not rotated, an explicit copy.  Alternating the two arrays and
+// copying explicitly keeps every step's results in fresh registers; with
+// in-place updates under the uniform branches the compiler re-merges the
+// registers after each step and moves half the amplitudes every time.
+template <int NR, int JB>
+__device__ __forceinline__ void rot_step(const float2 (&x)[NR], float2 (&y)[NR], int mode, float c, float sn) {
+  if (mode == 1) {  // Rx
+#pragma unroll
+    for (int m = 0; m < NR; ++m) {
+      if (m & (1 << JB)) continue;
+      const float2 a = x[m], b = x[m | (1 << JB)];
+      y[m] = make_float2(c * a.x + sn * b.y, c * a.y - sn * b.x);
+      y[m | (1 << JB)] = make_float2(sn * a.y + c * b.x, -sn * a.x + c * b.y);
+    }
+  } else if (mode == 2) {  // Ry
+#pragma unroll
+    for (int m = 0; m < NR; ++m) {
+      if (m & (1 << JB)) continue;
+      const float2 a = x[m], b = x[m | (1 << JB)];
+      y[m] = make_float2(c * a.x - sn * b.x, c * a.y - sn * b.y);
+      y[m | (1 << JB)] = make_float2(sn * a.x + c * b.x, sn * a.y + c * b.y);
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < NR; ++m) y[m] = make_float2(fresh(x[m].x), fresh(x[m].y));
+  }
+}
+
+// Steps JB .. R-1 of a window rotation, alternating a -> b -> a ...; the
+// result ends in v.
+template <int TQ, int JB>
+__device__ __forceinline__ void rot_chain(float2 (&a)[1 << (TQ - 8)], float2 (&b)[1 << (TQ - 8)],
+                                          float2 (&v)[1 << (TQ - 8)], const int* mode, const float* c,
+                                          const float* sn) {
+  constexpr int NR = 1 << (TQ - 8);
+  if constexpr (JB < TQ - 8) {
+    rot_step<NR, JB>(a, b, mode[JB], c[JB], sn[JB]);
+    rot_chain<TQ, JB + 1>(b, a, v, mode, c, sn);
+  } else {
+#pragma unroll
+    for (int m = 0; m < NR; ++m) v[m] = a[m];
+  }
+}
+
 // Rotations of one layer on the tile bits of window lo that are own bits not
 // yet rotated in this layer (done: mask of tile bits).  rx: Rx with
 // (c, s) = rc/rs[q] = (cos, sin) of half the angle; else Ry.
@@ -368,26 +421,23 @@ template <int TQ>
 __device__ __forceinline__ void wide_rotate(float2 (&v)[1 << (TQ - 8)], int lo, int own_lo, uint32_t& done,
                                             const WideCtx<TQ>& cx, bool rx, const float* rc, const float* rs) {
   constexpr int R = TQ - 8, NR = 1 << R;
+  int mode[R];
+  float c[R], sn[R];
 #pragma unroll
   for (int jb = 0; jb < R; ++jb) {
     const int bit = lo + jb;
+    mode[jb] = 0;
+    c[jb] = 1.0f;
+    sn[jb] = 0.0f;
     if (bit < own_lo || ((done >> bit) & 1u)) continue;
     done |= 1u << bit;
     const int q = cx.qubit_of(bit);
-    const float c = rc[q], sn = rs[q];
-#pragma unroll
-    for (int m = 0; m < NR; ++m) {
-      if (m & (1 << jb)) continue;
-      const float2 a = v[m], b = v[m | (1 << jb)];
-      if (rx) {
-        v[m] = make_float2(c * a.x + sn * b.y, c * a.y - sn * b.x);
-        v[m | (1 << jb)] = make_float2(sn * a.y + c * b.x, -sn * a.x + c * b.y);
-      } else {
-        v[m] = make_float2(c * a.x - sn * b.x, c * a.y - sn * b.y);
-        v[m | (1 << jb)] = make_float2(sn * a.x + c * b.x, sn * a.y + c * b.y);
-      }
-    }
+    mode[jb] = rx ? 1 : 2;
+    c[jb] = rc[q];
+    sn[jb] = rs[q];
   }
+  float2 w[NR];
+  rot_chain<TQ, 0>(v, w, v, mode, c, sn);
 }
 
 template <int TQ>

@@ -15,6 +15,7 @@ def build():
     s = s.replace("#include <hip/hip_runtime.h>", "")
     s = s.replace('#include "batch.h"', '#include "%s"' % os.path.join(ROOT, "ruqu_amd", "csrc", "batch.h"))
     s = re.sub(r"extern __shared__ float2 (\w+)\[\];", r"float2* \1 = (float2*)emu::lds;", s)
+    s = s.replace('asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(x));', "r = x;")  # register copy
     s = s.replace("__shared__ ", "static ")  # block-shared = one copy (blocks run one at a time)
     s += "\nnamespace rq { inline double emu_cut_of(uint64_t i, const BEdge* e, int ne) { return cut_of(i, e, ne); } }\n"
     inc = os.path.join(HERE, "batch_kernel_src.inc")
