@@ -27,7 +27,7 @@ namespace rq {
 
 __global__ void k_batch_eval(const BInst* insts, const BItem* items, int nitems, unsigned* counter,
                              const int32_t* done, const BEdge* edges, const BTerm* terms, const double* params,
-                             double* energy, uint32_t* amax);
+                             const float* ctab, double* energy, uint32_t* amax);
 template <int TQ>
 __global__ void k_wide_pass(const BInst* insts, const BItem* witems, const int32_t* wmap, const int32_t* wtile0,
                             const int64_t* wstate, const int32_t* done, const double* params, const float* ctab,
@@ -260,7 +260,7 @@ void build_plan(Batch& b) {
   b.tq = std::max(b.tq, (nmax + 1) / 2);  // two tile passes per layer
   b.ctab_total = 0;
   for (BInst& in : b.insts) {
-    if (in.kind == BK_QAOA && in.n > BATCH_LDS_QUBITS) {
+    if (in.kind == BK_QAOA) {  // C(i) tables for every QAOA state, LDS-resident ones included
       in.ctab_off = b.ctab_total;
       b.ctab_total += (int64_t)1 << in.n;
     } else {
@@ -378,9 +378,11 @@ int finalize(Batch& b) {
   RQ_HIP(hipMalloc((void**)&b.d_best_params, std::max<size_t>(b.params.size(), 1) * sizeof(double)));
   RQ_HIP(hipMalloc((void**)&b.d_ctab, std::max<int64_t>(b.ctab_total, 1) * sizeof(float)));
   for (const BInst& in : b.insts)
-    if (in.ctab_off >= 0)
-      hipLaunchKernelGGL(k_cut_table, dim3(1024), dim3(BATCH_THREADS), 0, b.st, b.d_ctab + in.ctab_off, in.n,
-                         b.d_edges + in.term_off, in.nterms);
+    if (in.ctab_off >= 0) {
+      const int64_t blocks = std::min<int64_t>(1024, std::max<int64_t>(1, ((int64_t)1 << in.n) / BATCH_THREADS));
+      hipLaunchKernelGGL(k_cut_table, dim3((unsigned)blocks), dim3(BATCH_THREADS), 0, b.st, b.d_ctab + in.ctab_off,
+                         in.n, b.d_edges + in.term_off, in.nterms);
+    }
   rc = upload(&b.d_out0, b.out0);
   if (!rc) rc = upload(&b.d_outr, b.outr);
   if (!rc) rc = upload_plan(b);
@@ -409,7 +411,7 @@ void enqueue_round(Batch& b, const RoundPlan& rp, unsigned* counter, const int32
   if (rp.nsmall) {
     const unsigned grid = (unsigned)std::min(b.nwg, rp.nsmall);
     hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BATCH_THREADS), (1 << BATCH_LDS_QUBITS) * 8, b.st, b.d_insts,
-                       b.d_small + rp.small_off, rp.nsmall, counter, done, b.d_edges, b.d_terms, b.d_params,
+                       b.d_small + rp.small_off, rp.nsmall, counter, done, b.d_edges, b.d_terms, b.d_params, b.d_ctab,
                        b.d_energy, b.d_amax);
   }
   if (!rp.nwide) return;

@@ -164,17 +164,17 @@ struct Shared {
 };
 
 // ---------------------------------------------------------------- QAOA ----
-__device__ void qaoa_lds(float2* s, const BInst& in, const BItem& it, const BEdge* __restrict__ edges,
+__device__ void qaoa_lds(float2* s, const BInst& in, const BItem& it, const float* __restrict__ ctab,
                          const double* __restrict__ params, Shared& sh, double* energy, uint32_t* amax) {
   const int n = in.n, p = in.p;
   const uint32_t N = 1u << n;
-  const BEdge* e = edges + in.term_off;
+  const float* __restrict__ ct = ctab + in.ctab_off;  // C(i), from k_cut_table
   const float c0 = plus_amp(n);
   for (uint32_t i = threadIdx.x; i < N; i += B) s[i] = make_float2(c0, 0.0f);
   __syncthreads();
   for (int l = 0; l < p; ++l) {
     const double gamma = param_of(params, in, it, l), beta = param_of(params, in, it, p + l);
-    for (uint32_t i = threadIdx.x; i < N; i += B) s[i] = phase_mul(s[i], gamma, cut_of(i, e, in.nterms), in.wsum);
+    for (uint32_t i = threadIdx.x; i < N; i += B) s[i] = phase_mul(s[i], gamma, (double)ct[i], in.wsum);
     __syncthreads();
     float c, sn;
     half_angle(beta, &c, &sn);  // Rx(2 beta): half angle beta
@@ -185,7 +185,7 @@ __device__ void qaoa_lds(float2* s, const BInst& in, const BItem& it, const BEdg
   for (uint32_t i = threadIdx.x; i < N; i += B) {
     float2 a = s[i];
     double pr = (double)a.x * a.x + (double)a.y * a.y;
-    acc += pr * cut_of(i, e, in.nterms);
+    acc += pr * (double)ct[i];
     better(bv, bi, (double)(a.x * a.x + a.y * a.y), i);
   }
   *energy = block_sum_all(acc, sh.red);
@@ -252,7 +252,8 @@ __global__ __launch_bounds__(B) void k_batch_eval(const BInst* __restrict__ inst
                                                   int nitems, unsigned* __restrict__ counter,
                                                   const int32_t* __restrict__ done, const BEdge* __restrict__ edges,
                                                   const BTerm* __restrict__ terms, const double* __restrict__ params,
-                                                  double* __restrict__ energy, uint32_t* __restrict__ amax) {
+                                                  const float* __restrict__ ctab, double* __restrict__ energy,
+                                                  uint32_t* __restrict__ amax) {
   extern __shared__ float2 lds_state[];
   __shared__ Shared sh;
   // Every branch of this loop is uniform across a wave (wave index and item
@@ -287,7 +288,7 @@ __global__ __launch_bounds__(B) void k_batch_eval(const BInst* __restrict__ inst
     }
     double en = 0.0;
     uint32_t am = 0;
-    if (in.kind == BK_QAOA) qaoa_lds(lds_state, in, it, edges, params, sh, &en, &am);
+    if (in.kind == BK_QAOA) qaoa_lds(lds_state, in, it, ctab, params, sh, &en, &am);
     else vqe_lds(lds_state, in, it, terms, params, sh, &en);
     if (wid == 0) {  // en and am are block-uniform: wave 0 stores them
       energy[it.out] = en;

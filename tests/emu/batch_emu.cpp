@@ -90,10 +90,12 @@ void run_grid(int grid, const std::function<void()>& body) {
 extern "C" {
 
 int emu_batch_eval(const rq::BInst* insts, const rq::BItem* items, int nitems, const int32_t* done,
-                   const rq::BEdge* edges, const rq::BTerm* terms, const double* params, double* energy,
-                   uint32_t* amax, int grid) {
+                   const rq::BEdge* edges, const rq::BTerm* terms, const double* params, const float* ctab,
+                   double* energy, uint32_t* amax, int grid) {
   unsigned counter = 0;
-  run_grid(grid, [&] { rq::k_batch_eval(insts, items, nitems, &counter, done, edges, terms, params, energy, amax); });
+  run_grid(grid, [&] {
+    rq::k_batch_eval(insts, items, nitems, &counter, done, edges, terms, params, ctab, energy, amax);
+  });
   return 0;
 }
 

@@ -35,7 +35,7 @@ def _emu():
     lib = C.CDLL(LIB)
     vp, i = C.c_void_p, C.c_int
     lib.emu_cut_table.argtypes = [vp, i, vp, i]
-    lib.emu_batch_eval.argtypes = [vp, vp, i] + [vp] * 6 + [i]
+    lib.emu_batch_eval.argtypes = [vp, vp, i] + [vp] * 7 + [i]
     lib.emu_wide_pass.argtypes = [i] + [vp] * 9 + [i, i, vp]
     lib.emu_wide_energy.argtypes = [i] + [vp] * 9 + [i, vp]
     lib.emu_wide_fold.argtypes = [i] + [vp] * 5 + [i, vp, vp]
@@ -96,8 +96,8 @@ def evaluate(batch, grid=3):
     amax = np.zeros(r["nouts"], dtype=np.uint32)
     done = np.zeros(len(insts), dtype=np.int32)
     if r["nsmall"]:
-        lib.emu_batch_eval(_p(insts), _p(r["small"]), r["nsmall"], _p(done), _p(ea), _p(ta), _p(pa), _p(energy),
-                           _p(amax), min(grid, r["nsmall"]))
+        lib.emu_batch_eval(_p(insts), _p(r["small"]), r["nsmall"], _p(done), _p(ea), _p(ta), _p(pa), _p(ctab),
+                           _p(energy), _p(amax), min(grid, r["nsmall"]))
     if r["nwide"]:
         states = np.zeros(max(r["state_amps"], 1), dtype=np.complex64)
         parts = np.zeros(r["ntiles"], dtype=PART)

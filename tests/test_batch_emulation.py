@@ -55,13 +55,13 @@ def test_batch_plan_order_and_layout():
     ins = d["insts"]
     assert list(ins["n"]) == [16, 6, 12, 14]
     assert list(ins["nparams"]) == [4, 2, 36, 6]
-    assert list(ins["ctab_off"]) == [0, -1, -1, 1 << 16]
+    assert list(ins["ctab_off"]) == [0, 1 << 16, -1, (1 << 16) + (1 << 6)]  # a C(i) table per QAOA instance
     assert np.isclose(ins[0]["wsum"], sum(w))
     assert len(d["params"]) == 4 + 2 + 36 + 6
     assert np.all(d["params"][:6] == 0.5) and np.all(d["params"][6:42] == 0.1) and np.all(d["params"][42:] == 0.5)
     r = batch_emu.plan(b, -1)
     tq = r["tq"]
-    assert tq == 12 and r["ctab_total"] == (1 << 16) + (1 << 14)
+    assert tq == 12 and r["ctab_total"] == (1 << 16) + (1 << 6) + (1 << 14)
     assert list(r["small"]["inst"]) == [2, 1]  # largest state first
     assert list(r["wide"]["inst"]) == [3, 0]  # most layers first: layer l launches a prefix
     assert list(r["wtile0"]) == [0, 1 << (14 - tq)]
