@@ -465,6 +465,12 @@ __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FS
     const uint64_t tile = blockIdx.x;
     for (int j = 0; j < st.nouter; ++j) outer |= ((tile >> j) & 1ull) << st.outer[j];
   }
+  // Wave priority: high while moving data (HBM loads and stores, LDS
+  // transposes), low while applying ops.  The tiles of a stage all do the
+  // same work, so their waves drift in step through the memory and the
+  // arithmetic sections; letting the data movement of a wave that starts or
+  // ends a tile go first keeps HBM busy while the other waves compute.
+  __builtin_amdgcn_s_setprio(3);
   Cx<T> v[FUSE_R];
   const int np = st.nphase;
   for (int p = 0; p < np; ++p) {
@@ -485,7 +491,9 @@ __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FS
       Unroll<FUSE_R, T>::gather_l(v, lds, mp, lane_lds_addr<T>(mp.tthr));
       __syncthreads();
     }
+    __builtin_amdgcn_s_setprio(0);
     apply_ops(v, ops, ph.op_begin, ph.op_end, pool, tid);
+    __builtin_amdgcn_s_setprio(3);
     if (p == np - 1) {
       if (ph.direct) {
         Unroll<FUSE_R, T>::scatter_g(v, a, mp);
