@@ -285,8 +285,8 @@ __device__ __forceinline__ void apply_one(const Amps<T>& x, Amps<T>& y, uint32_t
   const int kind = w & 7u, t0 = (w >> 3) & 3u, t1 = (w >> 5) & 3u, cl = (w >> 7) & 3u, cp = (w >> 9) & 31u,
             dl0 = (w >> 14) & 3u, dp0 = (w >> 16) & 63u, dl1 = (w >> 22) & 3u, dp1 = (w >> 24) & 63u;
   switch (kind) {
-    case FOP_U1:
-      RQ_P1(u1, t0, x, y, mp)
+    case FOP_U1:  // cloc/cpos of a 2x2: a tile-id bit that selects the second of two matrices
+      RQ_P1(u1, t0, x, y, (cl == PL_OUT && loc_val(cl, cp, tid)) ? mp + 4 : mp)
       break;
     case FOP_U2: {
       const int p0 = t0, p1 = t1;
@@ -511,14 +511,61 @@ __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FS
   }
 }
 
+// True when op `x` involves register bit b in any role.
+bool touches_reg_bit(const FOp& x, int b) {
+  switch (x.kind) {
+    case FOP_U1: return x.t0 == b;
+    case FOP_U2: return x.t0 == b || x.t1 == b;
+    case FOP_CX: return x.t0 == b || ((x.ctl_reg >> b) & 1u);
+    default:  // D1, D2, CZ
+      return (x.dloc0 == LOC_REG && x.dpos0 == b) ||
+             (x.kind != FOP_D1 && x.dloc1 == LOC_REG && x.dpos1 == b);
+  }
+}
+
 // Packs the plan's ops for the kernel: one 8-byte record each, control and
 // diagonal positions of outer qubits turned into tile-id (blockIdx) bits, and
 // every phase padded to an even record count with POP_NOP.  `phases` receives
-// the plan's phases with op ranges into `out`.
-int pack_ops(const FusedPlan& plan, std::vector<POp>& out, std::vector<FPhase>& phases) {
+// the plan's phases with op ranges into `out`; `pool` the matrix pool,
+// extended by the folded matrices below.
+//
+// Folding: a CNOT whose target is register bit b and whose control is absent
+// or a tile-id bit c (X_b, or X_b when c = 1), and a CZ of register bit b with
+// a tile-id bit c (Z_b when c = 1), commute with every op that does not
+// involve bit b, and c is constant over the tile.  Such an op is moved to the
+// next op of its phase that involves b when that op is an uncontrolled 2x2 on
+// b (M), and merged into it: M X or M Z is exact (a column swap or sign), and
+// with a control bit the 2x2 takes the pair [M, M X] or [M, M Z] and picks by
+// bit c.  The folded op then costs no register moves at all.
+int pack_ops(const FusedPlan& plan, std::vector<POp>& out, std::vector<FPhase>& phases, std::vector<rq_c64>& pool) {
   out.clear();
   out.reserve(plan.ops.size() + plan.phases.size());
   phases = plan.phases;
+  pool = plan.pool;
+  std::vector<int> fold_into(plan.ops.size(), -1);  // op -> index of the 2x2 it merges into
+  std::vector<int> folded_by(plan.ops.size(), -1);  // 2x2 -> the op merged into it
+  for (const FPhase& ph : plan.phases) {
+    for (int o = ph.op_begin; o < ph.op_end; ++o) {
+      const FOp& op = plan.ops[o];
+      int b = -1;
+      if (op.kind == FOP_CX && !op.ctl_reg && !op.ctl_thr) b = op.t0;
+      if (op.kind == FOP_CZ) {
+        if (op.dloc0 == LOC_REG && op.dloc1 == LOC_OUT) b = op.dpos0;
+        if (op.dloc1 == LOC_REG && op.dloc0 == LOC_OUT) b = op.dpos1;
+      }
+      if (b < 0) continue;
+      for (int q = o + 1; q < ph.op_end; ++q) {
+        const FOp& x = plan.ops[q];
+        if (!touches_reg_bit(x, b)) continue;
+        if (x.kind == FOP_U1 && x.t0 == b && !x.ctl_reg && !x.ctl_thr && !x.ctl_out && folded_by[q] < 0 &&
+            fold_into[q] < 0) {
+          fold_into[o] = q;
+          folded_by[q] = o;
+        }
+        break;
+      }
+    }
+  }
   for (const FStage& st : plan.stages) {
     auto tile_bit = [&](int q) {
       for (int j = 0; j < st.nouter; ++j)
@@ -539,7 +586,34 @@ int pack_ops(const FusedPlan& plan, std::vector<POp>& out, std::vector<FPhase>& 
       phases[ph].op_begin = (int32_t)out.size();
       for (int o = plan.phases[ph].op_begin; o < plan.phases[ph].op_end; ++o) {
         const FOp& op = plan.ops[o];
+        if (fold_into[o] >= 0) continue;  // merged into a later 2x2
         uint32_t cl = 0, cp = 0, dl0 = 0, dp0 = 0, dl1 = 0, dp1 = 0;
+        uint32_t mat = (uint32_t)op.mat;
+        if (folded_by[o] >= 0) {
+          const FOp& f = plan.ops[folded_by[o]];
+          const rq_c64* m = &plan.pool[op.mat];
+          rq_c64 mf[4];
+          if (f.kind == FOP_CX) {  // M X: columns swapped
+            mf[0] = m[1]; mf[1] = m[0]; mf[2] = m[3]; mf[3] = m[2];
+          } else {  // M Z: second column negated
+            mf[0] = m[0]; mf[1] = rq_c64{-m[1].re, -m[1].im}; mf[2] = m[2]; mf[3] = rq_c64{-m[3].re, -m[3].im};
+          }
+          int cq = -1;  // the condition's global qubit (none: always)
+          if (f.kind == FOP_CX && f.ctl_out) cq = __builtin_ctzll(f.ctl_out);
+          if (f.kind == FOP_CZ) cq = f.dloc0 == LOC_OUT ? f.dpos0 : f.dpos1;
+          mat = (uint32_t)pool.size();
+          if (cq < 0) {
+            pool.insert(pool.end(), mf, mf + 4);
+          } else {
+            const int j = tile_bit(cq);
+            if (j < 0 || j >= 32) return fail(RQ_ERR_ARGUMENT, "fused control outside the tile id");
+            pool.insert(pool.end(), m, m + 4);
+            pool.insert(pool.end(), mf, mf + 4);
+            cl = PL_OUT;
+            cp = (uint32_t)j;
+          }
+          if (pool.size() >= (size_t(1) << 32)) return fail(RQ_ERR_ARGUMENT, "fused matrix pool too large");
+        }
         if (op.kind == FOP_U1 && (op.ctl_reg || op.ctl_thr || op.ctl_out))
           return fail(RQ_ERR_ARGUMENT, "fused 1-qubit op with a control");
         if (op.kind == FOP_CX) {
@@ -563,7 +637,7 @@ int pack_ops(const FusedPlan& plan, std::vector<POp>& out, std::vector<FPhase>& 
         POp w;
         w.x = (uint32_t)op.kind | ((uint32_t)op.t0 << 3) | ((uint32_t)op.t1 << 5) | (cl << 7) | (cp << 9) |
               (dl0 << 14) | (dp0 << 16) | (dl1 << 22) | (dp1 << 24);
-        w.y = (uint32_t)op.mat;
+        w.y = mat;
         out.push_back(w);
       }
       if ((out.size() - (size_t)phases[ph].op_begin) & 1) out.push_back(POp{(uint32_t)POP_NOP, 0u});
@@ -577,8 +651,9 @@ template <class T>
 int launch_plan_t(State& s, const FusedPlan& plan) {
   std::vector<POp> pops;
   std::vector<FPhase> phases;
-  if (int rc = pack_ops(plan, pops, phases)) return rc;
-  const size_t nst = plan.stages.size(), nph = phases.size(), nop = pops.size(), npool = plan.pool.size();
+  std::vector<rq_c64> poolx;
+  if (int rc = pack_ops(plan, pops, phases, poolx)) return rc;
+  const size_t nst = plan.stages.size(), nph = phases.size(), nop = pops.size(), npool = poolx.size();
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t o_st = 0, o_ph = al(nst * sizeof(FStage)), o_op = o_ph + al(nph * sizeof(FPhase)),
                o_pool = o_op + al(nop * sizeof(POp)), total = o_pool + al(npool * sizeof(Cx<T>));
@@ -598,7 +673,7 @@ int launch_plan_t(State& s, const FusedPlan& plan) {
   std::memcpy(host.data() + o_ph, phases.data(), nph * sizeof(FPhase));
   std::memcpy(host.data() + o_op, pops.data(), nop * sizeof(POp));
   Cx<T>* pp = reinterpret_cast<Cx<T>*>(host.data() + o_pool);
-  for (size_t k = 0; k < npool; ++k) pp[k] = Cx<T>{(T)plan.pool[k].re, (T)plan.pool[k].im};
+  for (size_t k = 0; k < npool; ++k) pp[k] = Cx<T>{(T)poolx[k].re, (T)poolx[k].im};
   // The host buffer is pageable: wait for the copy before it goes out of scope.
   RQ_HIP(hipMemcpyAsync(s.prog, host.data(), total, hipMemcpyHostToDevice, s.stream));
   RQ_HIP(hipStreamSynchronize(s.stream));
