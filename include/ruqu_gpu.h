@@ -349,15 +349,24 @@ int rq_fusion_configure(int m, int c);
 /* Host-only: schedule size of gates[0..n) (no Measure/Reset). */
 int rq_fusion_plan_info(uint32_t num_qubits, const rq_gate* gates, size_t ngates, int m, int c, uint64_t* stages,
                         uint64_t* phases, uint64_t* ops);
-/* Host-only: executes the fused schedule on host amplitudes with the device
- * kernel's data mapping (planner check without a GPU). */
+/* Host-only: executes the program the fused kernel would receive (packed op
+ * records, folded gates, matrix-core phases at the current threshold, tuning
+ * key 4) on host amplitudes with the kernel's data mapping and lane layout
+ * (planner and packing check without a GPU). */
 int rq_fusion_emulate(uint32_t num_qubits, int precision, const rq_gate* gates, size_t ngates, int m, int c,
                       void* host_amps);
+/* Host-only: size of that program: op records, matrix-core phases and the
+ * composed unitaries they select from. */
+int rq_fusion_packed_info(uint32_t num_qubits, int precision, const rq_gate* gates, size_t ngates, int m, int c,
+                          uint64_t* records, uint64_t* mf_phases, uint64_t* mf_unitaries);
 
 /* Process-wide tuning knobs (no reference counterpart; set by the autotuner):
  * key 0 = streaming kernels' amplitude groups per thread (1, 2, 4, 8),
  * key 1 = non-temporal HBM access (0/1), key 2 = fused tile qubits m,
- * key 3 = fused low qubits c. */
+ * key 3 = fused low qubits c, key 4 = fused matrix-core threshold: complex-f32
+ * phases whose ops cost at least this many VALU instructions per wave run as
+ * one composed 16x16 unitary on the matrix cores (-1: never).
+ * rq_tune_get returns -2 for an unknown key. */
 int rq_tune_set(int key, int value);
 int rq_tune_get(int key);
 

@@ -9,7 +9,8 @@ the cached choice without timing again:
     non-temporal access of the exact per-gate and copy kernels
     (rq_tune_set keys 0 and 1).
   * tune_fused: tile qubits m and low qubits c of the fused schedule
-    (keys 2 and 3), on a random circuit of the caller's size.
+    (keys 2 and 3), then for complex-f32 states the matrix-core threshold
+    (key 4), on a random circuit of the caller's size.
   * tune_batch: LDS tile width of the batched engine's states above 13
     qubits (rq_batch_set_tile), on the caller's own batch.
 
@@ -28,7 +29,7 @@ from . import workloads
 from ._native import lib
 from .state import QuantumState, SimConfig, Simulator, check
 
-KEY_UNROLL, KEY_NONTEMPORAL, KEY_FUSED_M, KEY_FUSED_C = 0, 1, 2, 3
+KEY_UNROLL, KEY_NONTEMPORAL, KEY_FUSED_M, KEY_FUSED_C, KEY_FUSED_MF = 0, 1, 2, 3, 4
 
 
 def cache_path():
@@ -106,14 +107,18 @@ def tune_streaming(num_qubits=28, precision="c32", reps=10, use_cache=True):
     return res
 
 
-def tune_fused(num_qubits, precision="c32", depth=4, seed=7, candidates=None, use_cache=True):
-    """Picks the fused schedule's (m, c) by timing a random circuit of the
-    caller's width (depth layers) through rq_run_circuit(fusion=1)."""
+def tune_fused(num_qubits, precision="c32", depth=4, seed=7, candidates=None, mf_candidates=(-1, 512, 768, 1024),
+               use_cache=True):
+    """Picks the fused schedule's (m, c), and for complex-f32 states its
+    matrix-core threshold, by timing a random circuit of the caller's width
+    (depth layers) through rq_run_circuit(fusion=1)."""
     key = _key("fused", "%s-%d" % (precision, num_qubits))
     cached = _load().get(key) if use_cache else None
     if cached:
         set_knob(KEY_FUSED_M, cached["m"])
         set_knob(KEY_FUSED_C, cached["c"])
+        if "mf" in cached:
+            set_knob(KEY_FUSED_MF, cached["mf"])
         return cached
     if candidates is None:
         candidates = [(m, c) for m in (10, 11, 12) for c in (3, 4, 5, 6) if c < m and m <= num_qubits]
@@ -129,6 +134,15 @@ def tune_fused(num_qubits, precision="c32", depth=4, seed=7, candidates=None, us
     (m, c), best = min(times.items(), key=lambda kv: kv[1])
     check(lib().rq_fusion_configure(m, c))
     res = {"m": m, "c": c, "ms": best, "tried": {"%d,%d" % k: v for k, v in times.items()}}
+    if precision == "c32" and mf_candidates:
+        mf_times = {}
+        for mf in mf_candidates:
+            set_knob(KEY_FUSED_MF, mf)
+            Simulator.run_with_config(circ, cfg, state=st)
+            mf_times[mf] = Simulator.run_with_config(circ, cfg, state=st).metrics.device_time_ms
+        mf = min(mf_times, key=mf_times.get)
+        set_knob(KEY_FUSED_MF, mf)
+        res.update(mf=mf, ms=mf_times[mf], tried_mf={str(k): v for k, v in mf_times.items()})
     _store(key, res)
     return res
 

@@ -39,10 +39,14 @@ def sources():
 # SimplifyCFG's common-instruction sinking would merge those bodies into
 # pointer phis and force the array into scratch memory.  Its op dispatch is
 # wave-uniform (scalar op records): left unstructurized, the switches compile
-# to plain scalar branch trees instead of predicate-flag chains.
+# to plain scalar branch trees instead of predicate-flag chains.  Its
+# matrix-core phases accumulate in ordinary VGPRs: with accumulation registers
+# the amplitudes of every phase would live in them after the phase and the
+# VALU phases pay 32 extra moves each way.
 EXTRA = {
     "kernels_fused.hip": ["-mllvm", "-sink-common-insts=false",
-                          "-mllvm", "-structurizecfg-skip-uniform-regions=true"],
+                          "-mllvm", "-structurizecfg-skip-uniform-regions=true",
+                          "-mllvm", "-amdgpu-mfma-vgpr-form"],
 }
 
 

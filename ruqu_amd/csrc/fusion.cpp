@@ -214,6 +214,7 @@ int plan_fused(uint32_t nq, const rq_gate* gates, size_t ngates, const FusionCon
       for (int tb = 0; tb < m && popc(R) < FUSE_R_BITS; ++tb) R |= 1ull << st.S[tb];
 
       FPhase ph{};
+      ph.mf_off = -1;
       int nb = 0;
       int reg_of[64];
       for (int q = 0; q < 64; ++q) reg_of[q] = -1;
@@ -334,120 +335,6 @@ int run_fused(State& s, const rq_gate* gates, size_t ngates, const double* draws
 
 }  // namespace rq
 
-// ---------------------------------------------------------------------------
-// Host execution of a plan with the device kernel's data mapping (tile
-// gather, per-phase register/thread split, op semantics).  Used to check the
-// planner on machines without a GPU; the arithmetic is plain complex math in
-// the state's precision.
-// ---------------------------------------------------------------------------
-namespace rq {
-namespace {
-
-template <class T>
-struct HC {
-  T re, im;
-};
-template <class T>
-inline HC<T> hmul(HC<T> m, HC<T> a) {
-  return HC<T>{m.re * a.re - m.im * a.im, m.re * a.im + m.im * a.re};
-}
-template <class T>
-inline HC<T> hadd(HC<T> a, HC<T> b) {
-  return HC<T>{a.re + b.re, a.im + b.im};
-}
-
-inline int host_loc_bit(int loc, int pos, int k, uint32_t tid, uint64_t outer) {
-  if (loc == LOC_REG) return (k >> pos) & 1;
-  if (loc == LOC_THR) return (int)((tid >> pos) & 1u);
-  return (int)((outer >> pos) & 1ull);
-}
-
-template <class T>
-void emulate_plan(const FusedPlan& plan, uint32_t nq, HC<T>* a) {
-  std::vector<HC<T>> pool(plan.pool.size());
-  for (size_t k = 0; k < pool.size(); ++k) pool[k] = HC<T>{(T)plan.pool[k].re, (T)plan.pool[k].im};
-  for (const FStage& st : plan.stages) {
-    const int m = st.m, nt = m - FUSE_R_BITS;
-    std::vector<HC<T>> tile((size_t)1 << m);
-    for (uint64_t tl = 0; tl < (1ull << st.nouter); ++tl) {
-      uint64_t outer = 0;
-      for (int j = 0; j < st.nouter; ++j) outer |= ((tl >> j) & 1ull) << st.outer[j];
-      auto gidx = [&](uint64_t t) {
-        uint64_t g = outer;
-        for (int b = 0; b < m; ++b) g |= ((t >> b) & 1ull) << st.S[b];
-        return g;
-      };
-      for (uint64_t t = 0; t < tile.size(); ++t) tile[t] = a[gidx(t)];
-      for (int p = 0; p < st.nphase; ++p) {
-        const FPhase& ph = plan.phases[st.phase0 + p];
-        for (uint32_t tid = 0; tid < (1u << nt); ++tid) {
-          uint32_t tidx[FUSE_R];
-          HC<T> v[FUSE_R];
-          for (int k = 0; k < FUSE_R; ++k) {
-            uint32_t t = 0;
-            for (int j = 0; j < nt; ++j) t |= ((tid >> j) & 1u) << ph.thrt[j];
-            for (int b = 0; b < FUSE_R_BITS; ++b) t |= (uint32_t)((k >> b) & 1) << ph.regt[b];
-            tidx[k] = t;
-            v[k] = tile[t];
-          }
-          for (int o = ph.op_begin; o < ph.op_end; ++o) {
-            const FOp& op = plan.ops[o];
-            const HC<T>* mp = &pool[op.mat];
-            if (op.kind == FOP_U1) {
-              if ((tid & op.ctl_thr) != op.ctl_thr || (outer & op.ctl_out) != op.ctl_out) continue;
-              int bb = 1 << op.t0;
-              for (int k = 0; k < FUSE_R; ++k) {
-                if ((k & bb) || (k & (int)op.ctl_reg) != (int)op.ctl_reg) continue;
-                HC<T> x = v[k], y = v[k | bb];
-                v[k] = hadd(hmul(mp[0], x), hmul(mp[1], y));
-                v[k | bb] = hadd(hmul(mp[2], x), hmul(mp[3], y));
-              }
-            } else if (op.kind == FOP_CX) {
-              if ((tid & op.ctl_thr) != op.ctl_thr || (outer & op.ctl_out) != op.ctl_out) continue;
-              int bb = 1 << op.t0;
-              for (int k = 0; k < FUSE_R; ++k) {
-                if ((k & bb) || (k & (int)op.ctl_reg) != (int)op.ctl_reg) continue;
-                HC<T> x = v[k];
-                v[k] = v[k | bb];
-                v[k | bb] = x;
-              }
-            } else if (op.kind == FOP_CZ) {
-              for (int k = 0; k < FUSE_R; ++k)
-                if (host_loc_bit(op.dloc0, op.dpos0, k, tid, outer) & host_loc_bit(op.dloc1, op.dpos1, k, tid, outer))
-                  v[k] = HC<T>{-v[k].re, -v[k].im};
-            } else if (op.kind == FOP_U2) {
-              int b0 = 1 << op.t0, b1 = 1 << op.t1;
-              for (int k = 0; k < FUSE_R; ++k) {
-                if (k & (b0 | b1)) continue;
-                int idx[4] = {k, k | b1, k | b0, k | b0 | b1};
-                HC<T> x[4] = {v[idx[0]], v[idx[1]], v[idx[2]], v[idx[3]]};
-                for (int r = 0; r < 4; ++r) {
-                  HC<T> s{0, 0};
-                  for (int c = 0; c < 4; ++c) s = hadd(s, hmul(mp[r * 4 + c], x[c]));
-                  v[idx[r]] = s;
-                }
-              }
-            } else if (op.kind == FOP_D1) {
-              for (int k = 0; k < FUSE_R; ++k) v[k] = hmul(mp[host_loc_bit(op.dloc0, op.dpos0, k, tid, outer)], v[k]);
-            } else {
-              for (int k = 0; k < FUSE_R; ++k) {
-                int i = host_loc_bit(op.dloc0, op.dpos0, k, tid, outer) * 2 +
-                        host_loc_bit(op.dloc1, op.dpos1, k, tid, outer);
-                v[k] = hmul(mp[i], v[k]);
-              }
-            }
-          }
-          for (int k = 0; k < FUSE_R; ++k) tile[tidx[k]] = v[k];
-        }
-      }
-      for (uint64_t t = 0; t < tile.size(); ++t) a[gidx(t)] = tile[t];
-    }
-  }
-}
-
-}  // namespace
-}  // namespace rq
-
 extern "C" {
 
 // Plans gates[0..n) (no measurements) for a state of nq qubits with tile m and
@@ -467,8 +354,9 @@ int rq_fusion_plan_info(uint32_t nq, const rq_gate* gates, size_t ngates, int m,
 }
 
 // Runs the fused schedule of gates[0..n) on host amplitudes (rq_c64 or rq_c32
-// per precision) with the device kernel's data mapping.  Host only: used to
-// check the planner without a GPU.
+// per precision): the program the kernel would receive, with its data mapping
+// and the current matrix-core threshold.  Host only: used to check the
+// planner and the packing without a GPU.
 int rq_fusion_emulate(uint32_t nq, int precision, const rq_gate* gates, size_t ngates, int m, int c, void* amps) {
   rq::FusedPlan plan;
   rq::FusionConfig cfg;
@@ -476,9 +364,21 @@ int rq_fusion_emulate(uint32_t nq, int precision, const rq_gate* gates, size_t n
   cfg.c = c;
   int rc = rq::plan_fused(nq, gates, ngates, cfg, &plan);
   if (rc) return rc;
-  if (precision == RQ_C64) rq::emulate_plan<double>(plan, nq, (rq::HC<double>*)amps);
-  else rq::emulate_plan<float>(plan, nq, (rq::HC<float>*)amps);
-  return RQ_OK;
+  return rq::emulate_packed(plan, precision, rq::fusion_config().mf, amps);
+}
+
+// Size of the program the kernel would receive for gates[0..n): op records,
+// matrix-core phases and their composed unitaries (current matrix-core
+// threshold).  Host only.
+int rq_fusion_packed_info(uint32_t nq, int precision, const rq_gate* gates, size_t ngates, int m, int c,
+                          uint64_t* records, uint64_t* mf_phases, uint64_t* mf_unitaries) {
+  rq::FusedPlan plan;
+  rq::FusionConfig cfg;
+  cfg.m = m;
+  cfg.c = c;
+  int rc = rq::plan_fused(nq, gates, ngates, cfg, &plan);
+  if (rc) return rc;
+  return rq::packed_info(plan, precision, rq::fusion_config().mf, records, mf_phases, mf_unitaries);
 }
 
 // Sets the tile size (m qubits) and the count of low qubits always kept in a

@@ -50,6 +50,8 @@ struct FPhase {
   int32_t direct;                // 1: lanes sit on tile bits 0..5, access HBM directly
   int32_t regt[FUSE_R_BITS];     // tile bits holding register bits 0..3
   int32_t thrt[FUSE_MAX_M];      // tile bits holding thread-id bits 0..m-5
+  int32_t mf_off;                // -1: ops on the VALU; else pool offset of the phase's matrix-core operands
+  uint32_t mf_sel;               // matrix-core phase: the uniform bits that select its matrix (3 x 8 bits)
 };
 
 struct FStage {
@@ -72,6 +74,8 @@ struct FusionStats {
 struct FusionConfig {
   int m = 12;      // tile qubits
   int c = 5;       // lowest qubits always in the tile
+  int mf = 768;    // complex-f32 phases whose ops cost at least this many VALU instructions per wave run on
+                   // the matrix cores as one composed 16x16 unitary; -1: never
 };
 
 // Executes gates[0..ngates) with the fused schedule; Measure/Reset gates run
@@ -94,6 +98,16 @@ int plan_fused(uint32_t nq, const rq_gate* gates, size_t ngates, const FusionCon
 
 // Device launch of a planned segment (kernels_fused.hip).
 int launch_fused_plan(State& s, const FusedPlan& plan);
+
+// Host execution of the program the kernel receives for a plan (packed op
+// records, folded gates, matrix-core phases) with the kernel's data mapping
+// and lane layout; `amps` holds rq_c64 or rq_c32 values per precision.  For
+// checking the packing on machines without a GPU.
+int emulate_packed(const FusedPlan& plan, int precision, int mf_min, void* amps);
+// Size of the packed program: op records, matrix-core phases and their
+// composed unitaries.
+int packed_info(const FusedPlan& plan, int precision, int mf_min, uint64_t* records, uint64_t* mf_phases,
+                uint64_t* mf_unitaries);
 
 // Measurement helpers shared with the fused runner (state.cpp).
 int measure_state(State& s, uint32_t q, double draw, rq_measurement* out);

@@ -358,6 +358,64 @@ __device__ __forceinline__ void apply_ops(Amps<T>& v, const POp* __restrict__ op
   }
 }
 
+// Matrix-core phase (complex f32).  The host composes all ops of the phase
+// into one 16x16 complex unitary U over the register bits (one U per value of
+// the phase's uniform condition bits) and stores it as the real 32x32 matrix
+// [[Re U, -Im U], [Im U, Re U]] in the A-operand order of
+// v_mfma_f32_32x32x2_f32.  The phase's mapping swaps the roles of register
+// bit 3 and thread bit 5: lane bit 5 (h) holds register bit 3 and v[8c + j]
+// is chain c's amplitude j | h << 3, so lanes l and l + 32 hold the two halves
+// of one group of 16 amplitudes, 32 groups per chain and 2 chains per wave.
+// Per chain, D = A X with X[K][n] = real K of group n: step s takes K = 2s + h,
+// the real (s & 1 ? im : re) of amplitude (s >> 1) | h << 3, from lane n + 32h,
+// and the output row order puts real (v & 1) of amplitude (v >> 1) | h << 3
+// back in accumulator v of lane n + 32h.
+//   mf_sel: 3 bytes, each kind:2 | pos:6 — MF_OUT: tile-id bit pos,
+//   MF_WAVE: thread bit pos (6 or 7: the same in all lanes of a wave),
+//   MF_CHAIN: thread bit 5 of the phase's VALU mapping (the chain index c).
+// The variant index has bit i from byte i; variants are 1024 floats apart,
+// float (s >> 2) * 256 + lane * 4 + (s & 3) of a variant is lane's A value at step s.
+constexpr uint32_t MF_OUT = 1, MF_WAVE = 2, MF_CHAIN = 3;
+constexpr int MF_FLOATS = 1024;
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ void apply_mf(Cx<float> (&v)[FUSE_R], const float* __restrict__ w, uint32_t sel,
+                                         uint32_t tid) {
+  uint32_t vb = 0, cb = 0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const uint32_t d = (sel >> (8 * i)) & 255u, kind = d >> 6, pos = d & 63u;
+    if (kind == MF_OUT) vb |= ((blockIdx.x >> pos) & 1u) << i;
+    else if (kind == MF_WAVE) vb |= ((tid >> pos) & 1u) << i;
+    else if (kind == MF_CHAIN) cb = 1u << i;
+  }
+  vb = __builtin_amdgcn_readfirstlane(vb);
+  const uint32_t lane = tid & 63u;
+  const float4* __restrict__ w0 = reinterpret_cast<const float4*>(w + vb * MF_FLOATS) + lane;
+  const float4* __restrict__ w1 = reinterpret_cast<const float4*>(w + (vb | cb) * MF_FLOATS) + lane;
+  float a0[16], a1[16];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float4 x0 = w0[q * 64], x1 = w1[q * 64];
+    a0[4 * q] = x0.x; a0[4 * q + 1] = x0.y; a0[4 * q + 2] = x0.z; a0[4 * q + 3] = x0.w;
+    a1[4 * q] = x1.x; a1[4 * q + 1] = x1.y; a1[4 * q + 2] = x1.z; a1[4 * q + 3] = x1.w;
+  }
+  f32x16 d0 = {}, d1 = {};
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const float x0 = (s & 1) ? v[s >> 1].im : v[s >> 1].re;
+    const float x1 = (s & 1) ? v[8 + (s >> 1)].im : v[8 + (s >> 1)].re;
+    d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[s], x0, d0, 0, 0, 0);
+    d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[s], x1, d1, 0, 0, 0);
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    v[j] = Cx<float>{d0[2 * j], d0[2 * j + 1]};
+    v[8 + j] = Cx<float>{d1[2 * j], d1[2 * j + 1]};
+  }
+}
+
 // Offsets of one phase mapping: thread part and per-register-bit parts of the
 // tile index (treg) and of the global index (greg).
 struct Map {
@@ -492,7 +550,12 @@ __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FS
       __syncthreads();
     }
     __builtin_amdgcn_s_setprio(0);
-    apply_ops(v, ops, ph.op_begin, ph.op_end, pool, tid);
+    if constexpr (sizeof(T) == 4) {
+      if (ph.mf_off >= 0) apply_mf(v, reinterpret_cast<const float*>(pool + ph.mf_off), ph.mf_sel, tid);
+      else apply_ops(v, ops, ph.op_begin, ph.op_end, pool, tid);
+    } else {
+      apply_ops(v, ops, ph.op_begin, ph.op_end, pool, tid);
+    }
     __builtin_amdgcn_s_setprio(3);
     if (p == np - 1) {
       if (ph.direct) {
@@ -507,6 +570,172 @@ __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FS
     } else {
       Unroll<FUSE_R, T>::scatter_l(v, lds, mp, lane_lds_addr<T>(mp.tthr));
       __syncthreads();
+    }
+  }
+}
+
+// ---- matrix-core phases: host side ------------------------------------------
+
+// VALU instructions per wave of one op (16 amplitudes per thread): a 2x2 is
+// 8 pairs x 16 FP32 ops, a 4x4 twice that; diagonals one complex product
+// per amplitude; permutations and sign flips one move per word.
+int op_cost(const FOp& op) {
+  switch (op.kind) {
+    case FOP_U1: return 128;
+    case FOP_U2: return 256;
+    case FOP_D1:
+    case FOP_D2: return 64;
+    default: return 32;
+  }
+}
+
+// The uniform condition bits of a matrix-core phase (at most 3).
+struct MfBits {
+  int n = 0;
+  uint32_t kind[3] = {0, 0, 0}, pos[3] = {0, 0, 0};
+  int find(uint32_t k, uint32_t p) const {
+    for (int i = 0; i < n; ++i)
+      if (kind[i] == k && pos[i] == p) return i;
+    return -1;
+  }
+  bool add(uint32_t k, uint32_t p) {
+    if (find(k, p) >= 0) return true;
+    if (n == 3) return false;
+    kind[n] = k;
+    pos[n] = p;
+    ++n;
+    return true;
+  }
+  uint32_t sel() const {
+    uint32_t s = 0;
+    for (int i = 0; i < n; ++i) s |= ((kind[i] << 6) | pos[i]) << (8 * i);
+    return s;
+  }
+};
+
+// Uniform-bit key of a condition bit at (loc, pos) in a phase with nt thread
+// bits: thread bit 5 -> the chain, 6 and 7 -> the wave, a global qubit ->
+// its tile-id bit.  False when the bit differs between the 32 groups of a
+// chain (thread bits 0..4) or is not located.
+template <class TileBit>
+bool mf_key(int loc, int pos, int nt, const TileBit& tile_bit, uint32_t* k, uint32_t* p) {
+  if (loc == LOC_THR) {
+    if (pos == 5) { *k = MF_CHAIN; *p = 5; return true; }
+    if ((pos == 6 || pos == 7) && pos < nt) { *k = MF_WAVE; *p = (uint32_t)pos; return true; }
+    return false;
+  }
+  if (loc == LOC_OUT) {
+    const int j = tile_bit(pos);
+    if (j < 0 || j >= 32) return false;
+    *k = MF_OUT;
+    *p = (uint32_t)j;
+    return true;
+  }
+  return false;
+}
+
+// Collects the phase's uniform condition bits; false when the phase cannot
+// run on the matrix cores.
+template <class TileBit>
+bool mf_phase_bits(const FusedPlan& plan, const FPhase& ph, int nt, const TileBit& tile_bit, MfBits* bits) {
+  if (nt < 6) return false;
+  for (int o = ph.op_begin; o < ph.op_end; ++o) {
+    const FOp& op = plan.ops[o];
+    uint32_t k, p;
+    if (op.kind == FOP_U1 || op.kind == FOP_U2) {
+      if (op.ctl_reg || op.ctl_thr || op.ctl_out) return false;
+    } else if (op.kind == FOP_CX) {
+      const int nctl = __builtin_popcount(op.ctl_reg) + __builtin_popcount(op.ctl_thr) + __builtin_popcountll(op.ctl_out);
+      if (nctl > 1) return false;
+      if (op.ctl_thr && !(mf_key(LOC_THR, __builtin_ctz(op.ctl_thr), nt, tile_bit, &k, &p) && bits->add(k, p)))
+        return false;
+      if (op.ctl_out && !(mf_key(LOC_OUT, __builtin_ctzll(op.ctl_out), nt, tile_bit, &k, &p) && bits->add(k, p)))
+        return false;
+    } else {
+      if (op.dloc0 != LOC_REG && !(mf_key(op.dloc0, op.dpos0, nt, tile_bit, &k, &p) && bits->add(k, p))) return false;
+      if (op.kind != FOP_D1 && op.dloc1 != LOC_REG &&
+          !(mf_key(op.dloc1, op.dpos1, nt, tile_bit, &k, &p) && bits->add(k, p)))
+        return false;
+    }
+  }
+  return true;
+}
+
+// U[r][c] (row = output register index) of the phase's ops, in order, with
+// variant vi's values of the uniform bits.
+template <class TileBit>
+void mf_compose(const FusedPlan& plan, const FPhase& ph, int nt, const TileBit& tile_bit, const MfBits& bits,
+                uint32_t vi, rq_c64 (*U)[FUSE_R]) {
+  auto bitv = [&](int loc, int pos, int k) -> int {
+    if (loc == LOC_REG) return (k >> pos) & 1;
+    uint32_t kk = 0, pp = 0;
+    mf_key(loc, pos, nt, tile_bit, &kk, &pp);
+    return (int)((vi >> bits.find(kk, pp)) & 1u);
+  };
+  auto mul = [](rq_c64 a, rq_c64 b) { return rq_c64{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; };
+  auto add = [](rq_c64 a, rq_c64 b) { return rq_c64{a.re + b.re, a.im + b.im}; };
+  for (int col = 0; col < FUSE_R; ++col) {
+    rq_c64 x[FUSE_R];
+    for (int k = 0; k < FUSE_R; ++k) x[k] = rq_c64{k == col ? 1.0 : 0.0, 0.0};
+    for (int o = ph.op_begin; o < ph.op_end; ++o) {
+      const FOp& op = plan.ops[o];
+      const rq_c64* mp = &plan.pool[op.mat];
+      if (op.kind == FOP_U1) {
+        const int bb = 1 << op.t0;
+        for (int k = 0; k < FUSE_R; ++k) {
+          if (k & bb) continue;
+          const rq_c64 a = x[k], b = x[k | bb];
+          x[k] = add(mul(mp[0], a), mul(mp[1], b));
+          x[k | bb] = add(mul(mp[2], a), mul(mp[3], b));
+        }
+      } else if (op.kind == FOP_U2) {
+        const int b0 = 1 << op.t0, b1 = 1 << op.t1;
+        for (int k = 0; k < FUSE_R; ++k) {
+          if (k & (b0 | b1)) continue;
+          const int idx[4] = {k, k | b1, k | b0, k | b0 | b1};
+          const rq_c64 in[4] = {x[idx[0]], x[idx[1]], x[idx[2]], x[idx[3]]};
+          for (int r = 0; r < 4; ++r) {
+            rq_c64 acc{0.0, 0.0};
+            for (int c = 0; c < 4; ++c) acc = add(acc, mul(mp[r * 4 + c], in[c]));
+            x[idx[r]] = acc;
+          }
+        }
+      } else if (op.kind == FOP_CX) {
+        if (op.ctl_thr && !bitv(LOC_THR, __builtin_ctz(op.ctl_thr), 0)) continue;
+        if (op.ctl_out && !bitv(LOC_OUT, __builtin_ctzll(op.ctl_out), 0)) continue;
+        const int bb = 1 << op.t0;
+        for (int k = 0; k < FUSE_R; ++k) {
+          if ((k & bb) || (k & (int)op.ctl_reg) != (int)op.ctl_reg) continue;
+          std::swap(x[k], x[k | bb]);
+        }
+      } else if (op.kind == FOP_CZ) {
+        for (int k = 0; k < FUSE_R; ++k)
+          if (bitv(op.dloc0, op.dpos0, k) & bitv(op.dloc1, op.dpos1, k)) x[k] = rq_c64{-x[k].re, -x[k].im};
+      } else if (op.kind == FOP_D1) {
+        for (int k = 0; k < FUSE_R; ++k) x[k] = mul(mp[bitv(op.dloc0, op.dpos0, k)], x[k]);
+      } else {  // FOP_D2
+        for (int k = 0; k < FUSE_R; ++k)
+          x[k] = mul(mp[2 * bitv(op.dloc0, op.dpos0, k) + bitv(op.dloc1, op.dpos1, k)], x[k]);
+      }
+    }
+    for (int r = 0; r < FUSE_R; ++r) U[r][col] = x[r];
+  }
+}
+
+// A value of lane `lane` at MFMA step s: row m = lane % 32 and column
+// K = 2s + lane / 32 of the real matrix of U, in the row / column order of
+// apply_mf.
+void mf_operands(const rq_c64 (*U)[FUSE_R], float* w) {
+  for (int m = 0; m < 32; ++m) {
+    const int h = (m >> 2) & 1, vv = ((m >> 3) << 2) | (m & 3);
+    const int ao = (vv >> 1) | (h << 3), po = vv & 1;
+    for (int K = 0; K < 32; ++K) {
+      const int hk = K & 1, s = K >> 1;
+      const int ai = (s >> 1) | (hk << 3), pi = s & 1;
+      const rq_c64 u = U[ao][ai];
+      const double val = po == 0 ? (pi == 0 ? u.re : -u.im) : (pi == 0 ? u.im : u.re);
+      const int lane = m + 32 * hk;
+      w[(s >> 2) * 256 + lane * 4 + (s & 3)] = (float)val;
     }
   }
 }
@@ -537,14 +766,49 @@ bool touches_reg_bit(const FOp& x, int b) {
 // b (M), and merged into it: M X or M Z is exact (a column swap or sign), and
 // with a control bit the 2x2 takes the pair [M, M X] or [M, M Z] and picks by
 // bit c.  The folded op then costs no register moves at all.
-int pack_ops(const FusedPlan& plan, std::vector<POp>& out, std::vector<FPhase>& phases, std::vector<rq_c64>& pool) {
+//
+// Matrix-core phases (mf_min >= 0, complex f32): a phase whose ops cost at
+// least mf_min VALU instructions per wave and whose conditions are register
+// bits or uniform bits (MfBits) gets no records: its composed unitaries go to
+// the pool (aligned to 256 bytes) and its mapping swaps register bit 3 with
+// thread bit 5 (the layout apply_mf expects).
+int pack_ops(const FusedPlan& plan, int mf_min, std::vector<POp>& out, std::vector<FPhase>& phases,
+             std::vector<rq_c64>& pool) {
   out.clear();
   out.reserve(plan.ops.size() + plan.phases.size());
   phases = plan.phases;
   pool = plan.pool;
+  auto tile_bit_of = [](const FStage& st) {
+    return [&st](int q) {
+      for (int j = 0; j < st.nouter; ++j)
+        if (st.outer[j] == q) return j;
+      return -1;
+    };
+  };
+  std::vector<char> is_mf(plan.phases.size(), 0);
+  std::vector<MfBits> mfb(plan.phases.size());
+  if (mf_min >= 0) {
+    for (const FStage& st : plan.stages) {
+      const auto tile_bit = tile_bit_of(st);
+      const int nt = st.m - FUSE_R_BITS;
+      for (int ph = st.phase0; ph < st.phase0 + st.nphase; ++ph) {
+        const FPhase& f = plan.phases[ph];
+        if (f.op_end == f.op_begin) continue;
+        int cost = ((f.op_end - f.op_begin) & 1) ? 32 : 0;
+        for (int o = f.op_begin; o < f.op_end; ++o) cost += op_cost(plan.ops[o]);
+        if (cost < mf_min) continue;
+        MfBits b;
+        if (!mf_phase_bits(plan, f, nt, tile_bit, &b)) continue;
+        is_mf[ph] = 1;
+        mfb[ph] = b;
+      }
+    }
+  }
   std::vector<int> fold_into(plan.ops.size(), -1);  // op -> index of the 2x2 it merges into
   std::vector<int> folded_by(plan.ops.size(), -1);  // 2x2 -> the op merged into it
-  for (const FPhase& ph : plan.phases) {
+  for (size_t pi = 0; pi < plan.phases.size(); ++pi) {
+    if (is_mf[pi]) continue;
+    const FPhase& ph = plan.phases[pi];
     for (int o = ph.op_begin; o < ph.op_end; ++o) {
       const FOp& op = plan.ops[o];
       int b = -1;
@@ -567,11 +831,7 @@ int pack_ops(const FusedPlan& plan, std::vector<POp>& out, std::vector<FPhase>& 
     }
   }
   for (const FStage& st : plan.stages) {
-    auto tile_bit = [&](int q) {
-      for (int j = 0; j < st.nouter; ++j)
-        if (st.outer[j] == q) return j;
-      return -1;
-    };
+    const auto tile_bit = tile_bit_of(st);
     auto enc = [&](int loc, int pos, uint32_t* l, uint32_t* p) {
       if (loc == LOC_OUT) {
         const int j = tile_bit(pos);
@@ -584,6 +844,27 @@ int pack_ops(const FusedPlan& plan, std::vector<POp>& out, std::vector<FPhase>& 
     };
     for (int ph = st.phase0; ph < st.phase0 + st.nphase; ++ph) {
       phases[ph].op_begin = (int32_t)out.size();
+      if (is_mf[ph]) {
+        while (pool.size() % 32) pool.push_back(rq_c64{0.0, 0.0});
+        const size_t off = pool.size(), nv = (size_t)1 << mfb[ph].n;
+        if (off + nv * (MF_FLOATS / 2) >= ((size_t)1 << 31)) return fail(RQ_ERR_ARGUMENT, "fused matrix pool too large");
+        pool.resize(off + nv * (MF_FLOATS / 2), rq_c64{0.0, 0.0});
+        std::vector<float> w(MF_FLOATS);
+        for (uint32_t vi = 0; vi < nv; ++vi) {
+          rq_c64 U[FUSE_R][FUSE_R];
+          mf_compose(plan, plan.phases[ph], st.m - FUSE_R_BITS, tile_bit, mfb[ph], vi, U);
+          mf_operands(U, w.data());
+          for (int f = 0; f < MF_FLOATS; ++f) {
+            rq_c64& e = pool[off + vi * (MF_FLOATS / 2) + f / 2];
+            (f & 1 ? e.im : e.re) = (double)w[f];
+          }
+        }
+        phases[ph].op_end = (int32_t)out.size();
+        phases[ph].mf_off = (int32_t)off;
+        phases[ph].mf_sel = mfb[ph].sel();
+        std::swap(phases[ph].regt[3], phases[ph].thrt[5]);
+        continue;
+      }
       for (int o = plan.phases[ph].op_begin; o < plan.phases[ph].op_end; ++o) {
         const FOp& op = plan.ops[o];
         if (fold_into[o] >= 0) continue;  // merged into a later 2x2
@@ -652,7 +933,7 @@ int launch_plan_t(State& s, const FusedPlan& plan) {
   std::vector<POp> pops;
   std::vector<FPhase> phases;
   std::vector<rq_c64> poolx;
-  if (int rc = pack_ops(plan, pops, phases, poolx)) return rc;
+  if (int rc = pack_ops(plan, sizeof(T) == 4 ? fusion_config().mf : -1, pops, phases, poolx)) return rc;
   const size_t nst = plan.stages.size(), nph = phases.size(), nop = pops.size(), npool = poolx.size();
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t o_st = 0, o_ph = al(nst * sizeof(FStage)), o_op = o_ph + al(nph * sizeof(FPhase)),
@@ -691,7 +972,171 @@ int launch_plan_t(State& s, const FusedPlan& plan) {
   return RQ_OK;
 }
 
+
+// ---- host execution of the packed program -----------------------------------
+
+// One packed record on host amplitudes, with apply_one's semantics.
+template <class T>
+void host_apply_one(Cx<T> (&x)[FUSE_R], uint32_t w, const Cx<T>* mp, uint32_t tid, uint64_t tile) {
+  const int kind = w & 7u, t0 = (w >> 3) & 3u, t1 = (w >> 5) & 3u, cl = (w >> 7) & 3u, cp = (w >> 9) & 31u,
+            dl0 = (w >> 14) & 3u, dp0 = (w >> 16) & 63u, dl1 = (w >> 22) & 3u, dp1 = (w >> 24) & 63u;
+  auto mul = [](Cx<T> m, Cx<T> a) { return Cx<T>{m.re * a.re - m.im * a.im, m.re * a.im + m.im * a.re}; };
+  auto add = [](Cx<T> a, Cx<T> b) { return Cx<T>{a.re + b.re, a.im + b.im}; };
+  auto lv = [&](int loc, int pos) -> int {
+    return loc == PL_THR ? (int)((tid >> pos) & 1u) : (int)((tile >> pos) & 1ull);
+  };
+  auto bit = [&](int loc, int pos, int k) -> int { return loc == PL_REG ? (k >> pos) & 1 : lv(loc, pos); };
+  Cx<T> y[FUSE_R];
+  for (int k = 0; k < FUSE_R; ++k) y[k] = x[k];
+  if (kind == FOP_U1) {
+    const Cx<T>* m = (cl == PL_OUT && lv(cl, cp)) ? mp + 4 : mp;
+    const int bb = 1 << t0;
+    for (int k = 0; k < FUSE_R; ++k) {
+      if (k & bb) continue;
+      y[k] = add(mul(m[0], x[k]), mul(m[1], x[k | bb]));
+      y[k | bb] = add(mul(m[2], x[k]), mul(m[3], x[k | bb]));
+    }
+  } else if (kind == FOP_U2) {
+    const int b0 = 1 << t0, b1 = 1 << t1;
+    for (int k = 0; k < FUSE_R; ++k) {
+      if (k & (b0 | b1)) continue;
+      const int idx[4] = {k, k | b1, k | b0, k | b0 | b1};
+      for (int r = 0; r < 4; ++r) {
+        Cx<T> acc{0, 0};
+        for (int c = 0; c < 4; ++c) acc = add(acc, mul(mp[r * 4 + c], x[idx[c]]));
+        y[idx[r]] = acc;
+      }
+    }
+  } else if (kind == FOP_CX) {
+    const int bb = 1 << t0;
+    for (int k = 0; k < FUSE_R; ++k) {
+      const bool on = cl == PL_NONE || (cl == PL_REG ? ((k >> cp) & 1) : lv(cl, cp));
+      if (on) y[k] = x[k ^ bb];
+    }
+  } else if (kind == FOP_CZ) {
+    for (int k = 0; k < FUSE_R; ++k)
+      if (bit(dl0, dp0, k) & bit(dl1, dp1, k)) y[k] = Cx<T>{-x[k].re, -x[k].im};
+  } else if (kind == FOP_D1) {
+    for (int k = 0; k < FUSE_R; ++k) y[k] = mul(mp[bit(dl0, dp0, k)], x[k]);
+  } else if (kind == FOP_D2) {
+    for (int k = 0; k < FUSE_R; ++k) y[k] = mul(mp[2 * bit(dl0, dp0, k) + bit(dl1, dp1, k)], x[k]);
+  }  // POP_NOP: unchanged
+  for (int k = 0; k < FUSE_R; ++k) x[k] = y[k];
+}
+
+// apply_mf on the 64 lanes of one wave (regs[l] = lane l's amplitudes): the
+// 32x32x2 f32 matrix-core product spelled out (A[i][k] from lane i + 32k,
+// B[k][j] from lane j + 32k, D[i][j] to lane j + 32 ((i / 4) % 2), entry
+// (i % 4) + 4 (i / 8)), accumulated in FP64.
+void host_apply_mf(Cx<float> (*regs)[FUSE_R], const float* w, uint32_t sel, uint32_t wave, uint64_t tile) {
+  uint32_t vb = 0, cb = 0;
+  for (int i = 0; i < 3; ++i) {
+    const uint32_t d = (sel >> (8 * i)) & 255u, kind = d >> 6, pos = d & 63u;
+    if (kind == MF_OUT) vb |= (uint32_t)((tile >> pos) & 1u) << i;
+    else if (kind == MF_WAVE) vb |= (((wave * 64u) >> pos) & 1u) << i;
+    else if (kind == MF_CHAIN) cb = 1u << i;
+  }
+  Cx<float> out[64][FUSE_R];
+  for (int c = 0; c < 2; ++c) {
+    const float* wc = w + (size_t)(vb | (c ? cb : 0u)) * MF_FLOATS;
+    for (int i = 0; i < 32; ++i)
+      for (int j = 0; j < 32; ++j) {
+        double acc = 0.0;
+        for (int s = 0; s < 16; ++s)
+          for (int kk = 0; kk < 2; ++kk) {
+            const double a = wc[(s >> 2) * 256 + (i + 32 * kk) * 4 + (s & 3)];
+            const Cx<float>& xv = regs[j + 32 * kk][8 * c + (s >> 1)];
+            acc += a * (double)((s & 1) ? xv.im : xv.re);
+          }
+        const int lane = j + 32 * ((i >> 2) & 1), e = (i & 3) + 4 * (i >> 3);
+        Cx<float>& o = out[lane][8 * c + (e >> 1)];
+        (e & 1 ? o.im : o.re) = (float)acc;
+      }
+  }
+  for (int l = 0; l < 64; ++l)
+    for (int k = 0; k < FUSE_R; ++k) regs[l][k] = out[l][k];
+}
+
+template <class T>
+void emulate_packed_t(const FusedPlan& plan, const std::vector<POp>& pops, const std::vector<FPhase>& phases,
+                      const std::vector<rq_c64>& poolx, Cx<T>* a) {
+  std::vector<Cx<T>> pool(poolx.size());
+  for (size_t k = 0; k < pool.size(); ++k) pool[k] = Cx<T>{(T)poolx[k].re, (T)poolx[k].im};
+  for (const FStage& st : plan.stages) {
+    const int m = st.m, nt = m - FUSE_R_BITS;
+    const uint32_t nthr = 1u << nt;
+    std::vector<Cx<T>> tile((size_t)1 << m);
+    std::vector<Cx<T>> regs((size_t)nthr * FUSE_R);
+    std::vector<uint32_t> where((size_t)nthr * FUSE_R);
+    for (uint64_t tl = 0; tl < (1ull << st.nouter); ++tl) {
+      uint64_t outer = 0;
+      for (int j = 0; j < st.nouter; ++j) outer |= ((tl >> j) & 1ull) << st.outer[j];
+      auto gidx = [&](uint64_t t) {
+        uint64_t g = outer;
+        for (int b = 0; b < m; ++b) g |= ((t >> b) & 1ull) << st.S[b];
+        return g;
+      };
+      for (uint64_t t = 0; t < tile.size(); ++t) tile[t] = a[gidx(t)];
+      for (int p = 0; p < st.nphase; ++p) {
+        const FPhase& ph = phases[st.phase0 + p];
+        for (uint32_t tid = 0; tid < nthr; ++tid)
+          for (int k = 0; k < FUSE_R; ++k) {
+            uint32_t t = 0;
+            for (int j = 0; j < nt; ++j) t |= ((tid >> j) & 1u) << ph.thrt[j];
+            for (int b = 0; b < FUSE_R_BITS; ++b) t |= (uint32_t)((k >> b) & 1) << ph.regt[b];
+            where[(size_t)tid * FUSE_R + k] = t;
+            regs[(size_t)tid * FUSE_R + k] = tile[t];
+          }
+        if constexpr (sizeof(T) == 4) {
+          if (ph.mf_off >= 0) {
+            const float* w = reinterpret_cast<const float*>(pool.data() + ph.mf_off);
+            for (uint32_t wv = 0; wv < nthr / 64; ++wv)
+              host_apply_mf(reinterpret_cast<Cx<float>(*)[FUSE_R]>(regs.data() + (size_t)wv * 64 * FUSE_R), w,
+                            ph.mf_sel, wv, tl);
+          }
+        }
+        for (uint32_t tid = 0; tid < nthr; ++tid) {
+          Cx<T>(&x)[FUSE_R] = *reinterpret_cast<Cx<T>(*)[FUSE_R]>(regs.data() + (size_t)tid * FUSE_R);
+          for (int o = ph.op_begin; o < ph.op_end; ++o) host_apply_one(x, pops[o].x, pool.data() + pops[o].y, tid, tl);
+        }
+        for (size_t e = 0; e < regs.size(); ++e) tile[where[e]] = regs[e];
+      }
+      for (uint64_t t = 0; t < tile.size(); ++t) a[gidx(t)] = tile[t];
+    }
+  }
+}
+
 }  // namespace
+
+int emulate_packed(const FusedPlan& plan, int precision, int mf_min, void* amps) {
+  std::vector<POp> pops;
+  std::vector<FPhase> phases;
+  std::vector<rq_c64> poolx;
+  if (int rc = pack_ops(plan, precision == RQ_C64 ? -1 : mf_min, pops, phases, poolx)) return rc;
+  if (precision == RQ_C64) emulate_packed_t<double>(plan, pops, phases, poolx, static_cast<Cx<double>*>(amps));
+  else emulate_packed_t<float>(plan, pops, phases, poolx, static_cast<Cx<float>*>(amps));
+  return RQ_OK;
+}
+
+int packed_info(const FusedPlan& plan, int precision, int mf_min, uint64_t* records, uint64_t* mf_phases,
+                uint64_t* mf_unitaries) {
+  std::vector<POp> pops;
+  std::vector<FPhase> phases;
+  std::vector<rq_c64> poolx;
+  if (int rc = pack_ops(plan, precision == RQ_C64 ? -1 : mf_min, pops, phases, poolx)) return rc;
+  uint64_t n = 0, u = 0;
+  for (const FPhase& ph : phases) {
+    if (ph.mf_off < 0) continue;
+    ++n;
+    int bits = 0;
+    for (int i = 0; i < 3; ++i) bits += ((ph.mf_sel >> (8 * i)) & 255u) != 0;
+    u += 1ull << bits;
+  }
+  if (records) *records = pops.size();
+  if (mf_phases) *mf_phases = n;
+  if (mf_unitaries) *mf_unitaries = u;
+  return RQ_OK;
+}
 
 int launch_fused_plan(State& s, const FusedPlan& plan) {
   if (plan.stages.empty()) return RQ_OK;

@@ -832,7 +832,8 @@ extern "C" {
 
 // Tuning knobs (no reference counterpart).  key 0: streaming unroll (1, 2, 4,
 // 8); key 1: non-temporal loads/stores (0/1); key 2: fused tile qubits m;
-// key 3: fused low qubits c.
+// key 3: fused low qubits c; key 4: fused matrix-core threshold (VALU
+// instructions per wave of a complex-f32 phase's ops; -1: never).
 int rq_tune_set(int key, int value) {
   switch (key) {
     case 0:
@@ -850,6 +851,10 @@ int rq_tune_set(int key, int value) {
       if (value < 0 || value > FUSE_MAX_M) return fail(RQ_ERR_ARGUMENT, "low qubits out of range");
       fusion_config().c = value;
       return RQ_OK;
+    case 4:
+      if (value < -1 || value > (1 << 20)) return fail(RQ_ERR_ARGUMENT, "matrix-core threshold out of range");
+      fusion_config().mf = value;
+      return RQ_OK;
     default:
       return fail(RQ_ERR_ARGUMENT, "unknown tuning key");
   }
@@ -861,7 +866,8 @@ int rq_tune_get(int key) {
     case 1: return stream_tune().nontemporal ? 1 : 0;
     case 2: return fusion_config().m;
     case 3: return fusion_config().c;
-    default: return -1;
+    case 4: return fusion_config().mf;
+    default: return -2;
   }
 }
 

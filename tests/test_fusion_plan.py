@@ -49,3 +49,52 @@ def test_fused_plan_reduces_passes():
     stages, phases, ops = plan_info(30, arr, 12, 5)
     assert stages < len(arr) / 10
     assert ops <= len(arr)
+
+
+def packed_info(n, arr, m, c, prec):
+    """(op records, matrix-core phases, composed unitaries) of the packed program."""
+    r, f, u = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    assert lib().rq_fusion_packed_info(n, 0 if prec == "c64" else 1, ptr(arr), len(arr), m, c, C.byref(r),
+                                       C.byref(f), C.byref(u)) == 0
+    return r.value, f.value, u.value
+
+
+@pytest.fixture
+def mf_threshold():
+    """Sets the matrix-core threshold (tuning key 4) for one test and restores it."""
+    old = lib().rq_tune_get(4)
+    yield lambda v: lib().rq_tune_set(4, v)
+    lib().rq_tune_set(4, old)
+
+
+@pytest.mark.parametrize("thr", [-1, 0, 512])
+@pytest.mark.parametrize("n,depth,seed", [(10, 6, 3), (12, 6, 4), (14, 4, 5)])
+def test_packed_program_matrix_core_phases(mf_threshold, thr, n, depth, seed):
+    """The packed program (folds, matrix-core phases with the 32x32x2 lane
+    layout spelled out on the host) against the oracle."""
+    mf_threshold(thr)
+    arr = workloads.random_circuit(n, depth, seed).to_array()
+    m = 12 if n >= 12 else 10
+    recs, mf, _ = packed_info(n, arr, m, 5, "c32")
+    if thr == 0:
+        assert mf > 0
+    if thr == -1:
+        assert mf == 0
+    want, _, _ = orc.run_circuit(n, arr)
+    assert max_abs(emulate(n, arr, m, 5, "c32"), want) < 1e-5
+    # complex-f64 states never use the matrix-core phases
+    assert packed_info(n, arr, m, 5, "c64")[1] == 0
+
+
+def test_matrix_core_phases_with_uniform_conditions(mf_threshold):
+    """Mixed circuits put CNOT controls and CZ bits on tile-id and wave bits:
+    matrix-core phases then select among several composed unitaries."""
+    mf_threshold(0)
+    phases = unitaries = 0
+    for seed in range(6):
+        arr = mixed_circuit(13, 4, 1077 + seed).to_array()
+        _, f, u = packed_info(13, arr, 12, 5, "c32")
+        phases, unitaries = phases + f, unitaries + u
+        want, _, _ = orc.run_circuit(13, arr)
+        assert max_abs(emulate(13, arr, 12, 5, "c32"), want) < 1e-5
+    assert phases > 0 and unitaries > phases
