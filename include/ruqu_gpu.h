@@ -212,6 +212,26 @@ typedef struct {
 int rq_grover_run(uint32_t num_qubits, const uint64_t* targets, size_t num_targets, int64_t iterations,
                   int precision, int with_seed, const double* draws, rq_grover_result* out, rq_state** state_out);
 
+/* ---- seeded RNG (state.rs:68: StdRng::seed_from_u64; rand 0.8.6,
+ * rand_chacha 0.3.1, rand_core 0.6.4 per the reference's Cargo.lock) ------
+ * The draws of a seeded QuantumState / Simulator: rq_stdrng_next_f64 gives
+ * the stream of rng.gen::<f64>() (state.rs:302, simulator.rs:187-214). */
+typedef struct rq_stdrng {
+  uint32_t key[8];   /* ChaCha12 key (seed_from_u64's PCG32 expansion) */
+  uint64_t block;    /* next block counter */
+  uint32_t buf[64];  /* 4 blocks of output words */
+  uint32_t index;    /* next word in buf (64: empty) */
+} rq_stdrng;
+void rq_stdrng_seed(rq_stdrng* r, uint64_t seed);
+uint64_t rq_stdrng_next_u64(rq_stdrng* r);
+uint32_t rq_stdrng_next_u32(rq_stdrng* r);
+void rq_stdrng_next_f64(rq_stdrng* r, double* out, size_t n);
+/* Simulator::run_shots' draws (simulator.rs:122-127): shot s uses a StdRng
+ * seeded with base_seed + s (wrapping); out[s*per_shot + i] is its i-th f64. */
+void rq_stdrng_shot_draws(uint64_t base_seed, size_t shots, size_t per_shot, double* out);
+/* The ChaCha block function (16 input words, `rounds` rounds) under the RNG. */
+void rq_chacha_block(const uint32_t* in, int rounds, uint32_t* out);
+
 /* ---- measurement (state.rs:288-352) ------------------------------------ */
 int rq_measure(rq_state* s, uint32_t qubit, double draw, rq_measurement* out);
 /* state.rs:329-335: draws[q] is consumed by qubit q, qubit 0 first. */

@@ -7,6 +7,7 @@ Gate, PauliString, Hamiltonian) so host code and tests read like the
 reference's own.
 """
 from .gates import Gate, GateKind, Hamiltonian, PauliOp, PauliString, QuantumCircuit
+from .rng import StdRng
 from .state import (
     ArgumentError,
     CircuitError,
@@ -32,5 +33,5 @@ __all__ = [
     "Gate", "GateKind", "Hamiltonian", "PauliOp", "PauliString", "QuantumCircuit", "ArgumentError", "CircuitError",
     "DeviceError", "InvalidQubitIndex", "InvalidStateVector", "MeasurementOutcome", "MemoryAllocationFailed",
     "NoiseModel", "QuantumError", "QuantumState", "QuantumStateF32", "QubitLimitExceeded", "ShotResult", "SimConfig",
-    "SimulationMetrics", "SimulationResult", "Simulator", "StateCheckpoint",
+    "SimulationMetrics", "SimulationResult", "Simulator", "StateCheckpoint", "StdRng",
 ]

@@ -86,6 +86,12 @@ SIGNATURES = {
     "rq_reset_qubit": (_i, [_vp, _u32, _d]),
     "rq_measurement_record": (_sz, [_vp, _vp, _sz]),
     "rq_sample_shots": (_i, [_vp, _vp, _sz, _vp, C.POINTER(_u64)]),
+    "rq_stdrng_seed": (None, [_vp, _u64]),
+    "rq_stdrng_next_u64": (_u64, [_vp]),
+    "rq_stdrng_next_u32": (_u32, [_vp]),
+    "rq_stdrng_next_f64": (None, [_vp, _vp, _sz]),
+    "rq_stdrng_shot_draws": (None, [_u64, _sz, _sz, _vp]),
+    "rq_chacha_block": (None, [_vp, _i, _vp]),
     "rq_probabilities": (_i, [_vp, _vp]),
     "rq_probability_of_qubit": (_i, [_vp, _u32, _pd]),
     "rq_expectation": (_i, [_vp, _vp, _sz, _pd]),

@@ -14,6 +14,7 @@ import numpy as np
 
 from . import _native
 from ._native import lib
+from .rng import state_rng
 from .state import QuantumState, _as_prec, check
 
 
@@ -51,12 +52,12 @@ def optimal_iterations(num_qubits, num_targets):  # grover.rs:69-74
 
 
 def run_grover(config, precision="c64", draws=None):  # grover.rs:98-183
-    """draws: num_qubits uniforms for the final measurement (default: drawn
-    from numpy's PCG64 seeded with config.seed)."""
+    """draws: num_qubits uniforms for the final measurement (default: the
+    stream of StdRng seeded with config.seed, as the reference's state)."""
     n = config.num_qubits
     t = np.ascontiguousarray(config.target_states, dtype=np.uint64)
-    if draws is None:
-        draws = np.random.default_rng(config.seed).random(n)
+    if draws is None:  # grover.rs:118-120: the state's StdRng, or entropy without a seed
+        draws = state_rng(config.seed).random(n)
     d = np.ascontiguousarray(draws, dtype=np.float64)
     if d.size < n:
         raise ValueError("run_grover needs %d measurement draws, got %d" % (n, d.size))

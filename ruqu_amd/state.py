@@ -7,9 +7,10 @@ explicit copies.
 
 Randomness: the reference draws measurement uniforms from a seeded StdRng
 inside the state (state.rs:302).  Here every measuring call accepts explicit
-draws; when none are given they come from numpy's PCG64 seeded with the
-state's seed (a different stream from rand's ChaCha, so seeded outcomes are
-self-consistent but not those of the reference).
+draws; when none are given they come from the same stream: rand 0.8.6's
+StdRng seeded with the state's seed (ruqu_amd.rng, restated in the native
+library), or OS entropy for an unseeded state.  Seeded runs therefore
+reproduce the reference's outcomes.
 """
 import ctypes as C
 import time
@@ -18,6 +19,7 @@ import numpy as np
 
 from . import _native
 from ._native import MEAS_DTYPE, Measurement, Metrics, Noise, lib, ptr
+from .rng import shot_draws, state_rng
 from .gates import GateKind, QuantumCircuit
 
 C64, C32 = 0, 1
@@ -107,7 +109,7 @@ class QuantumState:
     def __init__(self, handle, precision, seed=None):
         self._h = C.c_void_p(handle)
         self._prec = precision
-        self._rng = np.random.default_rng(seed)
+        self._rng = state_rng(seed)
 
     # ---- construction (state.rs:31-96) --------------------------------------
     @classmethod
@@ -468,8 +470,7 @@ class Simulator:
         nmr = _count_draws(arr)
         per_shot = nmr + (0 if has_meas else n)
         if draws is None:
-            rng = np.random.default_rng(base)
-            draws = rng.random((shots, per_shot))
+            draws = shot_draws(base, shots, per_shot)  # shot s: StdRng(base + s), simulator.rs:122-127
         draws = np.ascontiguousarray(draws, dtype=np.float64).reshape(shots, per_shot)
         counts = {}
         gate_total = 0
