@@ -14,7 +14,8 @@
 //
 // RNG: the reference draws uniforms from rand 0.8 StdRng (ChaCha12), which is
 // not vendored; every draw is therefore an explicit input here ("fed the
-// reference's uniform draws").  Parity of the seeded stream itself is unpinned.
+// reference's uniform draws").  The seeded stream itself is restated in
+// stdrng_oracle.py and pinned there by the RFC 7539 and PCG32 vectors.
 
 #include <cmath>
 #include <cstdint>
