@@ -148,7 +148,8 @@ def plain_baseline(insts, plain_iters, lr=0.1):
     return rp
 
 
-def evaluate(predictors, insts, plain_iters, refine_iters=0, lr=0.1, tol=0.01, log=print, plain=None):
+def evaluate(predictors, insts, plain_iters, refine_iters=0, lr=0.1, tol=0.01, log=print, plain=None,
+             rescue_iters=0):
     """Plain batched QAOA vs the warm start on the same instances.
 
     predictors: one predictor (an object with predict(graphs, ps)) or a
@@ -159,6 +160,9 @@ def evaluate(predictors, insts, plain_iters, refine_iters=0, lr=0.1, tol=0.01, l
     expected cut is below (1 - tol) x the plain result's (within 1 % of the
     plain-optimized energy; approximation ratios share the exact MaxCut
     denominator); the fallback runs are timed as a separate batch.
+    rescue_iters > 0 first gives the missing instances that many plain
+    updates from their chosen angles (one batch of just those instances);
+    only the ones still below the bar fall back.
     Instances/s = count / wall time of each pipeline."""
     if not isinstance(predictors, (list, tuple)):
         predictors = [predictors]
@@ -182,8 +186,20 @@ def evaluate(predictors, insts, plain_iters, refine_iters=0, lr=0.1, tol=0.01, l
         rw = run_plain(insts, refine_iters + 1, lr, init=best_a)
         best_e = np.maximum(best_e, rw["best"])
         t_eval += rw["wall_s"]
-    ratio_warm = best_e / cmax
+    ratio_raw = best_e / cmax
     ok = best_e >= (1.0 - tol) * rp["best"]
+    ok_raw = ok.copy()
+    t_rescue, rescued = 0.0, 0
+    miss = [i for i in range(N) if not ok[i]]
+    if rescue_iters > 0 and miss:
+        rr = run_plain([insts[i] for i in miss], rescue_iters + 1, lr, init=[best_a[i] for i in miss])
+        t_rescue = rr["wall_s"]
+        for j, i in enumerate(miss):
+            if rr["best"][j] > best_e[i]:
+                best_e[i] = rr["best"][j]
+        ok = best_e >= (1.0 - tol) * rp["best"]
+        rescued = int(np.sum(ok) - np.sum(ok_raw))
+    ratio_warm = best_e / cmax
     fb = [i for i in range(N) if not ok[i]]
     t_fb = 0.0
     if fb:
@@ -200,24 +216,27 @@ def evaluate(predictors, insts, plain_iters, refine_iters=0, lr=0.1, tol=0.01, l
         "plain_inst_per_s": N / rp["wall_s"],
         "warm_predict_s": t_pred,
         "warm_eval_s": t_eval,
+        "rescue_iterations": rescue_iters,
+        "rescue_s": t_rescue,
+        "rescued": rescued,
         "fallback_s": t_fb,
         "fallbacks": len(fb),
-        "warm_inst_per_s": N / (t_pred + t_eval + t_fb),
+        "warm_inst_per_s": N / (t_pred + t_eval + t_rescue + t_fb),
         "mean_ratio_plain": float(np.mean(ratio_plain)),
-        "mean_ratio_warm_raw": float(np.mean(ratio_warm)),
+        "mean_ratio_warm_raw": float(np.mean(ratio_raw)),
         "mean_ratio_warm_final": float(np.mean(final)),
-        "within_1pct_energy": float(np.mean(ok)),
-        "within_1pt_ratio": float(np.mean(ratio_warm >= ratio_plain - 0.01)),
+        "within_1pct_energy": float(np.mean(ok_raw)),
+        "within_1pt_ratio": float(np.mean(ratio_raw >= ratio_plain - 0.01)),
         "chosen_fraction": [float(np.mean(chosen == c)) for c in range(len(predictors))],
         "min_final_over_plain": float(np.min(final / ratio_plain)),
     }
     res["speedup"] = res["warm_inst_per_s"] / res["plain_inst_per_s"]
     if log:
         log("plain %.1f inst/s (ratio %.4f) | warm[%d] %.1f inst/s (final ratio %.4f, raw %.4f; %.1f%% within 1%% of "
-            "plain energy, %d fallbacks) -> %.2fx" % (res["plain_inst_per_s"], res["mean_ratio_plain"], len(predictors),
-                                                     res["warm_inst_per_s"], res["mean_ratio_warm_final"],
-                                                     res["mean_ratio_warm_raw"], 100 * res["within_1pct_energy"],
-                                                     len(fb), res["speedup"]))
+            "plain energy, %d rescued by %d updates, %d fallbacks) -> %.2fx" % (
+                res["plain_inst_per_s"], res["mean_ratio_plain"], len(predictors), res["warm_inst_per_s"],
+                res["mean_ratio_warm_final"], res["mean_ratio_warm_raw"], 100 * res["within_1pct_energy"], rescued,
+                rescue_iters, len(fb), res["speedup"]))
     return res
 
 

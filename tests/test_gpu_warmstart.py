@@ -33,3 +33,9 @@ def test_warm_start_gain():
     assert ens["speedup"] >= 5.0
     assert alone["mean_ratio_warm_raw"] >= start["mean_ratio_warm_raw"] + 0.01
     assert alone["within_1pct_energy"] >= start["within_1pct_energy"] + 0.1
+    # rescue updates: the same raw candidates, fewer fallbacks, still >= 99 % of plain everywhere
+    res = W.evaluate([net, W.DefaultInit(), const], insts, 16, log=print, plain=base, rescue_iters=6)
+    assert res["within_1pct_energy"] == ens["within_1pct_energy"]
+    assert res["rescued"] > 0 and res["fallbacks"] == ens["fallbacks"] - res["rescued"]
+    assert res["min_final_over_plain"] >= 0.99 - 1e-12
+    assert res["mean_ratio_warm_final"] >= ens["mean_ratio_warm_final"] - 1e-3
