@@ -333,6 +333,9 @@ int rq_batch_result(rq_batch* b, uint32_t id, double* best, uint64_t* best_index
 int rq_batch_results(rq_batch* b, double* best, uint64_t* best_index, uint32_t* iterations, int* converged,
                      double* params, double* best_params);
 int rq_batch_set_params(rq_batch* b, uint32_t id, const double* params);
+/* Every instance's parameters at once: params holds them back to back in
+ * instance order (count = the batch's total parameter count); one upload. */
+int rq_batch_set_all_params(rq_batch* b, const double* params, size_t count);
 uint32_t rq_batch_size(const rq_batch* b);
 /* Exact MaxCut value of every QAOA instance (device brute force over all 2^n
  * bitstrings, FP64 weights; 0 for VQE instances): the denominator of

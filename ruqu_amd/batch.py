@@ -27,6 +27,7 @@ _native.register("rq_batch_result", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(
                                               C.POINTER(C.c_uint32), C.POINTER(C.c_int), C.c_void_p, C.c_void_p,
                                               C.c_void_p])
 _native.register("rq_batch_set_params", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p])
+_native.register("rq_batch_set_all_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t])
 _native.register("rq_batch_size", C.c_uint32, [C.c_void_p])
 _native.register("rq_batch_last_ms", C.c_double, [C.c_void_p])
 _native.register("rq_batch_set_tile", C.c_int, [C.c_void_p, C.c_int])
@@ -127,6 +128,12 @@ class Batch:
     def set_params(self, i, params):
         a = np.ascontiguousarray(params, dtype=np.float64)
         check(lib().rq_batch_set_params(self._h, i, ptr(a)))
+
+    def set_all_params(self, params):
+        """Every instance's parameters (a list in instance order) in one upload."""
+        a = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.float64).ravel() for p in params])
+                                 if len(params) else np.zeros(0), dtype=np.float64)
+        check(lib().rq_batch_set_all_params(self._h, ptr(a) if a.size else None, a.size))
 
     def maxcut(self):
         """Exact MaxCut value of every QAOA instance (device brute force)."""

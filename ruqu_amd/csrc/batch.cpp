@@ -684,6 +684,18 @@ int rq_batch_set_params(rq_batch* h, uint32_t id, const double* params) {
   return RQ_OK;
 }
 
+int rq_batch_set_all_params(rq_batch* h, const double* params, size_t count) {
+  Batch* b = (Batch*)h;
+  if (!b || (!params && count)) return fail(RQ_ERR_ARGUMENT, "null pointer");
+  if (count != b->params.size())
+    return fail(RQ_ERR_ARGUMENT, "expected " + std::to_string(b->params.size()) + " parameters, got " +
+                                     std::to_string(count));
+  std::copy(params, params + count, b->params.begin());
+  if (b->ready && count)
+    RQ_HIP(hipMemcpy(b->d_params, params, count * sizeof(double), hipMemcpyHostToDevice));
+  return RQ_OK;
+}
+
 // LDS tile width of the wide (n > 13) states: 2^tq amplitudes per
 // workgroup.  Needs ceil(n_max / 2) <= tq <= 13 (two passes per layer).
 int rq_batch_set_tile(rq_batch* h, int tq) {

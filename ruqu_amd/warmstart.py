@@ -173,15 +173,22 @@ def evaluate(predictors, insts, plain_iters, refine_iters=0, lr=0.1, tol=0.01, l
     best_e = np.full(N, -np.inf)
     best_a = [None] * N
     chosen = np.zeros(N, dtype=np.int32)
+    b = None  # one batch for every candidate: new angles are one upload
     for ci, pr in enumerate(predictors):
         t0 = time.perf_counter()
         cand = predict(pr, insts)
         t_pred += time.perf_counter() - t0
-        e, t, _ = evaluate_at(insts, cand)
-        t_eval += t
+        t0 = time.perf_counter()
+        if b is None:
+            b = _batch(insts, cand)
+        else:
+            b.set_all_params([np.concatenate([np.asarray(g), np.asarray(bt)]) for g, bt in cand])
+        e, _ = b.evaluate()
+        t_eval += time.perf_counter() - t0
         for i in range(N):
             if e[i] > best_e[i]:
                 best_e[i], best_a[i], chosen[i] = e[i], cand[i], ci
+    del b  # frees the candidates' device states before the next batches
     if refine_iters > 0:
         rw = run_plain(insts, refine_iters + 1, lr, init=best_a)
         best_e = np.maximum(best_e, rw["best"])

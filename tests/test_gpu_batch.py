@@ -151,3 +151,29 @@ def test_batch_24_qubits_tile_widths_agree():
         b.set_tile(tq)
         got[tq] = b.evaluate()
     assert abs(got[12][0][0] - got[13][0][0]) <= 1e-5 * (1 + sum(w))
+
+
+def test_set_all_params_equals_per_instance_updates():
+    qc, vc = qaoa_cases(), vqe_cases()
+
+    def build():
+        b = Batch()
+        for n, e, w, p, g, be in qc:
+            b.add_qaoa(n, e, w, p, g, be)
+        for n, terms, prm in vc:
+            b.add_vqe(n, 4, terms, prm)
+        return b
+
+    rng = np.random.default_rng(123)
+    new = [rng.uniform(0, math.pi, 2 * p) for *_, p, _, _ in qc] + [rng.uniform(-1, 1, n * 4) for n, _, _ in vc]
+    one, bulk = build(), build()
+    one.evaluate()
+    bulk.evaluate()
+    for i, prm in enumerate(new):
+        one.set_params(i, prm)
+    bulk.set_all_params(new)
+    e1, a1 = one.evaluate()
+    e2, a2 = bulk.evaluate()
+    assert np.array_equal(e1, e2) and np.array_equal(a1, a2)
+    with pytest.raises(Exception):
+        bulk.set_all_params(new[:-1])
