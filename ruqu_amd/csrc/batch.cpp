@@ -37,7 +37,8 @@ __global__ void k_wide_energy(const BInst* insts, const BItem* witems, const int
                               const float2* states, int tq, BPart* parts);
 __global__ void k_wide_fold(const BInst* insts, const BItem* witems, const int32_t* wtile0, const int32_t* done,
                             const BPart* parts, int tq, double* energy, uint32_t* amax);
-__global__ void k_cut_table(float* ctab, int n, const BEdge* e, int ne);
+__global__ void k_cut_tables(float* ctab, const BInst* insts, const BEdge* edges, const BCutChunk* chunks,
+                             uint32_t nchunks);
 __global__ void k_maxcut(int n, const BEdge* e, int ne, unsigned long long* out);
 __global__ void k_zero_u32(unsigned* p, int n);
 
@@ -163,6 +164,7 @@ struct Batch {
   double* d_params = nullptr;
   float* d_ctab = nullptr;
   int64_t ctab_total = 0;
+  BCutChunk* d_cutchunks = nullptr;
   // plan: rounds 0..nrounds-1 of an optimizer iteration, then the base round
   int nrounds = 0;
   std::vector<RoundPlan> rounds;
@@ -377,12 +379,21 @@ int finalize(Batch& b) {
   if (rc) return rc;
   RQ_HIP(hipMalloc((void**)&b.d_best_params, std::max<size_t>(b.params.size(), 1) * sizeof(double)));
   RQ_HIP(hipMalloc((void**)&b.d_ctab, std::max<int64_t>(b.ctab_total, 1) * sizeof(float)));
-  for (const BInst& in : b.insts)
-    if (in.ctab_off >= 0) {
-      const int64_t blocks = std::min<int64_t>(1024, std::max<int64_t>(1, ((int64_t)1 << in.n) / BATCH_THREADS));
-      hipLaunchKernelGGL(k_cut_table, dim3((unsigned)blocks), dim3(BATCH_THREADS), 0, b.st, b.d_ctab + in.ctab_off,
-                         in.n, b.d_edges + in.term_off, in.nterms);
+  {  // every cut table in one launch
+    std::vector<BCutChunk> chunks;
+    for (size_t k = 0; k < b.insts.size(); ++k)
+      if (b.insts[k].ctab_off >= 0) {
+        const int64_t N = (int64_t)1 << b.insts[k].n;
+        for (int64_t c = 0; c * BATCH_CUT_CHUNK < N; ++c) chunks.push_back(BCutChunk{(int32_t)k, (uint32_t)c});
+      }
+    if (chunks.size() >= ((size_t)1 << 32)) return fail(RQ_ERR_ARGUMENT, "batch cut tables too large");
+    if (!chunks.empty()) {
+      rc = upload(&b.d_cutchunks, chunks);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_cut_tables, grid_blocks(chunks.size()), dim3(BATCH_THREADS), 0, b.st, b.d_ctab, b.d_insts,
+                         b.d_edges, b.d_cutchunks, (uint32_t)chunks.size());
     }
+  }
   rc = upload(&b.d_out0, b.out0);
   if (!rc) rc = upload(&b.d_outr, b.outr);
   if (!rc) rc = upload_plan(b);
@@ -464,7 +475,7 @@ void free_batch(Batch& b) {
   void* ptrs[] = {b.d_insts,  b.d_edges,  b.d_terms,  b.d_params, b.d_ctab,     b.d_small,   b.d_wide,
                   b.d_wtile0, b.d_wmap,   b.d_vmap,   b.d_wstate, b.d_states, b.d_parts,   b.d_energy,  b.d_amax,
                   b.d_counters, b.d_out0, b.d_outr,   b.o.iter,   b.o.hlen,     b.o.conv,    b.o.done,
-                  b.o.hist,   b.o.best,   b.o.best_idx, b.d_best_params, b.d_zero_done};
+                  b.o.hist,   b.o.best,   b.o.best_idx, b.d_best_params, b.d_zero_done, b.d_cutchunks};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (b.st) (void)hipStreamDestroy(b.st);

@@ -14,6 +14,7 @@ constexpr int BATCH_MIN_TILE_QUBITS = 9;
 constexpr int BATCH_MAX_TILE_QUBITS = 13;
 constexpr int BATCH_MAX_QUBITS = 24;       // two tile passes per layer need n <= 2 * tile qubits
 constexpr int BATCH_THREADS = 256;
+constexpr int BATCH_CUT_CHUNK = 4096;      // cut-table entries per workgroup of k_cut_tables
 
 enum BatchKind : int32_t { BK_QAOA = 0, BK_VQE = 1 };
 
@@ -33,6 +34,13 @@ struct BInst {
 struct BEdge {
   uint32_t i, j;
   double w;
+};
+
+// One workgroup of k_cut_tables: entries [chunk * BATCH_CUT_CHUNK, ...) of
+// instance inst's cut table.
+struct BCutChunk {
+  int32_t inst;
+  uint32_t chunk;
 };
 
 struct BTerm {

@@ -633,11 +633,21 @@ __global__ __launch_bounds__(B) void k_wide_fold(const BInst* __restrict__ insts
   }
 }
 
-// Cut table of a QAOA instance held in HBM: C(i) for every bitstring.
-__global__ __launch_bounds__(B) void k_cut_table(float* __restrict__ ctab, int n, const BEdge* __restrict__ e, int ne) {
-  const uint64_t N = 1ull << n;
-  for (uint64_t i = (uint64_t)blockIdx.x * B + threadIdx.x; i < N; i += (uint64_t)gridDim.x * B)
-    ctab[i] = (float)cut_of(i, e, ne);
+// Cut tables of every QAOA instance of a batch, C(i) for every bitstring,
+// in one launch: workgroup w (rows of the grid in blockIdx.y; the last row
+// may run past nchunks) fills BATCH_CUT_CHUNK entries of one table.
+__global__ __launch_bounds__(B) void k_cut_tables(float* __restrict__ ctab, const BInst* __restrict__ insts,
+                                                  const BEdge* __restrict__ edges,
+                                                  const BCutChunk* __restrict__ chunks, uint32_t nchunks) {
+  const uint64_t w = (uint64_t)blockIdx.y * gridDim.x + blockIdx.x;
+  if (w >= nchunks) return;
+  const BCutChunk ch = chunks[w];
+  const BInst& in = insts[ch.inst];
+  const uint64_t N = 1ull << in.n, first = (uint64_t)ch.chunk * BATCH_CUT_CHUNK;
+  const uint64_t end = N < first + BATCH_CUT_CHUNK ? N : first + BATCH_CUT_CHUNK;
+  float* __restrict__ t = ctab + in.ctab_off;
+  const BEdge* __restrict__ e = edges + in.term_off;
+  for (uint64_t i = first + threadIdx.x; i < end; i += B) t[i] = (float)cut_of(i, e, in.nterms);
 }
 
 // Maximum cut weight over all bitstrings; cut weights are non-negative, so

@@ -53,6 +53,8 @@ void run_grid(int grid, const std::function<void()>& body) {
   emu::xchg.assign(B, 0.0);
   blockDim.x = B;
   gridDim.x = grid;
+  gridDim.y = 1;
+  blockIdx.y = 0;  // one-row grids
   emu::cur_body = &body;
   std::vector<ucontext_t> ctx(B);
   std::vector<char> fin(B);
@@ -127,12 +129,9 @@ int emu_wide_fold(int grid, const rq::BInst* insts, const rq::BItem* wi, const i
   return 0;
 }
 
-void emu_cut_table(float* ctab, int n, const rq::BEdge* e, int ne) {
-  threadIdx.x = 0;
-  blockIdx.x = 0;
-  blockDim.x = 1;
-  gridDim.x = 1;
-  const uint64_t N = 1ull << n;
-  for (uint64_t i = 0; i < N; ++i) ctab[i] = (float)rq::emu_cut_of(i, e, ne);
+int emu_cut_tables(float* ctab, const rq::BInst* insts, const rq::BEdge* edges, const rq::BCutChunk* chunks,
+                   int nchunks) {
+  run_grid(nchunks, [&] { rq::k_cut_tables(ctab, insts, edges, chunks, (uint32_t)nchunks); });
+  return 0;
 }
 }

@@ -34,7 +34,7 @@ def _emu():
         build_emu.build()
     lib = C.CDLL(LIB)
     vp, i = C.c_void_p, C.c_int
-    lib.emu_cut_table.argtypes = [vp, i, vp, i]
+    lib.emu_cut_tables.argtypes = [vp, vp, vp, vp, i]
     lib.emu_batch_eval.argtypes = [vp, vp, i] + [vp] * 7 + [i]
     lib.emu_wide_pass.argtypes = [i] + [vp] * 9 + [i, i, vp]
     lib.emu_wide_energy.argtypes = [i] + [vp] * 9 + [i, vp]
@@ -87,11 +87,12 @@ def evaluate(batch, grid=3):
     lib = _emu()
     insts, ea, ta, pa = d["insts"], d["edges"], d["terms"], d["params"]
     ctab = np.zeros(max(r["ctab_total"], 1), dtype=np.float32)
-    for k in range(len(insts)):
-        off = int(insts[k]["ctab_off"])
-        if off >= 0:
-            lib.emu_cut_table(_p(ctab[off:]), int(insts[k]["n"]), _p(ea[int(insts[k]["term_off"]):]),
-                              int(insts[k]["nterms"]))
+    # the launch shape of batch.cpp: BATCH_CUT_CHUNK = 4096 entries per workgroup
+    chunks = np.array([(k, c) for k in range(len(insts)) if int(insts[k]["ctab_off"]) >= 0
+                       for c in range(-(-(1 << int(insts[k]["n"])) // 4096))],
+                      dtype=np.dtype([("inst", "<i4"), ("chunk", "<u4")]))
+    if len(chunks):
+        lib.emu_cut_tables(_p(ctab), _p(insts), _p(ea), _p(chunks), len(chunks))
     energy = np.zeros(r["nouts"], dtype=np.float64)
     amax = np.zeros(r["nouts"], dtype=np.uint32)
     done = np.zeros(len(insts), dtype=np.int32)

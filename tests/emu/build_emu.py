@@ -17,7 +17,6 @@ def build():
     s = re.sub(r"extern __shared__ float2 (\w+)\[\];", r"float2* \1 = (float2*)emu::lds;", s)
     s = s.replace('asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(x));', "r = x;")  # register copy
     s = s.replace("__shared__ ", "static ")  # block-shared = one copy (blocks run one at a time)
-    s += "\nnamespace rq { inline double emu_cut_of(uint64_t i, const BEdge* e, int ne) { return cut_of(i, e, ne); } }\n"
     inc = os.path.join(HERE, "batch_kernel_src.inc")
     with open(inc, "w") as f:
         f.write(s)
