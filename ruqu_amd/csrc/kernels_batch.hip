@@ -369,29 +369,52 @@ __device__ __forceinline__ float fresh(float x) {
   return r;
 }
 
-// One bit of a window rotation, x -> y (distinct register arrays): Rx or Ry
-// with (c, s) = (cos, sin) of half the angle on register bit JB, or, when the
-// bit is not rotated, an explicit copy.  Alternating the two arrays and
-// copying explicitly keeps every step's results in fresh registers; with
-// in-place updates under the uniform branches the compiler re-merges the
-// registers after each step and moves half the amplitudes every time.
+// One bit of a window rotation, x -> y (distinct register arrays), with the
+// larger of (c, s) = (cos, sin) of half the angle factored out of the 2x2
+// (the factor goes into the pass's uniform scale, see rot_factor), so that
+// a pair costs 4 FMAs with a ratio k of magnitude <= 1:
+//   mode 1: Rx = c [[1, -ik], [-ik, 1]], k = s / c
+//   mode 2: Ry = c [[1, -k], [k, 1]],    k = s / c
+//   mode 3: Rx = -is [[k, 1], [1, k]] with outputs (b + ika, a + ikb), k = c / s
+//   mode 4: Ry = s [[k, -1], [1, k]],    k = c / s
+// mode 0: the bit is not rotated, an explicit copy.  Alternating the two
+// arrays and copying explicitly keeps every step's results in fresh
+// registers; with in-place updates under the uniform branches the compiler
+// re-merges the registers after each step and moves half the amplitudes
+// every time.
 template <int NR, int JB>
-__device__ __forceinline__ void rot_step(const float2 (&x)[NR], float2 (&y)[NR], int mode, float c, float sn) {
-  if (mode == 1) {  // Rx
+__device__ __forceinline__ void rot_step(const float2 (&x)[NR], float2 (&y)[NR], int mode, float k) {
+  if (mode == 1) {
 #pragma unroll
     for (int m = 0; m < NR; ++m) {
       if (m & (1 << JB)) continue;
       const float2 a = x[m], b = x[m | (1 << JB)];
-      y[m] = make_float2(c * a.x + sn * b.y, c * a.y - sn * b.x);
-      y[m | (1 << JB)] = make_float2(sn * a.y + c * b.x, -sn * a.x + c * b.y);
+      y[m] = make_float2(a.x + k * b.y, a.y - k * b.x);
+      y[m | (1 << JB)] = make_float2(b.x + k * a.y, b.y - k * a.x);
     }
-  } else if (mode == 2) {  // Ry
+  } else if (mode == 2) {
 #pragma unroll
     for (int m = 0; m < NR; ++m) {
       if (m & (1 << JB)) continue;
       const float2 a = x[m], b = x[m | (1 << JB)];
-      y[m] = make_float2(c * a.x - sn * b.x, c * a.y - sn * b.y);
-      y[m | (1 << JB)] = make_float2(sn * a.x + c * b.x, sn * a.y + c * b.y);
+      y[m] = make_float2(a.x - k * b.x, a.y - k * b.y);
+      y[m | (1 << JB)] = make_float2(b.x + k * a.x, b.y + k * a.y);
+    }
+  } else if (mode == 3) {
+#pragma unroll
+    for (int m = 0; m < NR; ++m) {
+      if (m & (1 << JB)) continue;
+      const float2 a = x[m], b = x[m | (1 << JB)];
+      y[m] = make_float2(b.x - k * a.y, b.y + k * a.x);
+      y[m | (1 << JB)] = make_float2(a.x - k * b.y, a.y + k * b.x);
+    }
+  } else if (mode == 4) {
+#pragma unroll
+    for (int m = 0; m < NR; ++m) {
+      if (m & (1 << JB)) continue;
+      const float2 a = x[m], b = x[m | (1 << JB)];
+      y[m] = make_float2(k * a.x - b.x, k * a.y - b.y);
+      y[m | (1 << JB)] = make_float2(a.x + k * b.x, a.y + k * b.y);
     }
   } else {
 #pragma unroll
@@ -399,16 +422,39 @@ __device__ __forceinline__ void rot_step(const float2 (&x)[NR], float2 (&y)[NR],
   }
 }
 
+// Factored form of a rotation by (c, s) = (cos, sin) of half its angle:
+// mode offset (0: c factored out, 2: s), ratio k and the factor f that the
+// pass's scale takes (Rx: c or -is; Ry: c or s).
+struct RotFactor {
+  float k;
+  int big_s;
+  float2 fx, fy;  // factor of Rx, of Ry
+};
+__device__ __forceinline__ RotFactor rot_factor(double c, double s) {
+  RotFactor r;
+  if (fabs(c) >= fabs(s)) {
+    r.big_s = 0;
+    r.k = (float)(s / c);
+    r.fx = make_float2((float)c, 0.0f);
+    r.fy = make_float2((float)c, 0.0f);
+  } else {
+    r.big_s = 1;
+    r.k = (float)(c / s);
+    r.fx = make_float2(0.0f, (float)-s);
+    r.fy = make_float2((float)s, 0.0f);
+  }
+  return r;
+}
+
 // Steps JB .. R-1 of a window rotation, alternating a -> b -> a ...; the
 // result ends in v.
 template <int TQ, int JB>
 __device__ __forceinline__ void rot_chain(float2 (&a)[1 << (TQ - 8)], float2 (&b)[1 << (TQ - 8)],
-                                          float2 (&v)[1 << (TQ - 8)], const int* mode, const float* c,
-                                          const float* sn) {
+                                          float2 (&v)[1 << (TQ - 8)], const int* mode, const float* k) {
   constexpr int NR = 1 << (TQ - 8);
   if constexpr (JB < TQ - 8) {
-    rot_step<NR, JB>(a, b, mode[JB], c[JB], sn[JB]);
-    rot_chain<TQ, JB + 1>(b, a, v, mode, c, sn);
+    rot_step<NR, JB>(a, b, mode[JB], k[JB]);
+    rot_chain<TQ, JB + 1>(b, a, v, mode, k);
   } else {
 #pragma unroll
     for (int m = 0; m < NR; ++m) v[m] = a[m];
@@ -416,29 +462,30 @@ __device__ __forceinline__ void rot_chain(float2 (&a)[1 << (TQ - 8)], float2 (&b
 }
 
 // Rotations of one layer on the tile bits of window lo that are own bits not
-// yet rotated in this layer (done: mask of tile bits).  rx: Rx with
-// (c, s) = rc/rs[q] = (cos, sin) of half the angle; else Ry.
+// yet rotated in this layer (done: mask of tile bits).  rx: Rx, else Ry, in
+// the factored form of rf[q]; the factors multiply into scale (the same in
+// every thread: every amplitude of the state gets the same rotations).
 template <int TQ>
 __device__ __forceinline__ void wide_rotate(float2 (&v)[1 << (TQ - 8)], int lo, int own_lo, uint32_t& done,
-                                            const WideCtx<TQ>& cx, bool rx, const float* rc, const float* rs) {
+                                            const WideCtx<TQ>& cx, bool rx, const RotFactor* rf, float2& scale) {
   constexpr int R = TQ - 8, NR = 1 << R;
   int mode[R];
-  float c[R], sn[R];
+  float k[R];
 #pragma unroll
   for (int jb = 0; jb < R; ++jb) {
     const int bit = lo + jb;
     mode[jb] = 0;
-    c[jb] = 1.0f;
-    sn[jb] = 0.0f;
+    k[jb] = 0.0f;
     if (bit < own_lo || ((done >> bit) & 1u)) continue;
     done |= 1u << bit;
-    const int q = cx.qubit_of(bit);
-    mode[jb] = rx ? 1 : 2;
-    c[jb] = rc[q];
-    sn[jb] = rs[q];
+    const RotFactor& f = rf[cx.qubit_of(bit)];
+    mode[jb] = (rx ? 1 : 2) + 2 * f.big_s;
+    k[jb] = f.k;
+    const float2 g = rx ? f.fx : f.fy;
+    scale = make_float2(scale.x * g.x - scale.y * g.y, scale.x * g.y + scale.y * g.x);
   }
   float2 w[NR];
-  rot_chain<TQ, 0>(v, w, v, mode, c, sn);
+  rot_chain<TQ, 0>(v, w, v, mode, k);
 }
 
 template <int TQ>
@@ -451,7 +498,7 @@ __global__ __launch_bounds__(B) void k_wide_pass(const BInst* __restrict__ insts
                                                  float2* __restrict__ states, int pass, BPart* __restrict__ parts) {
   constexpr int R = TQ - 8, NR = 1 << R;
   extern __shared__ float2 lds[];  // 2^TQ + 2^(TQ-R) slots
-  __shared__ float rc[2][BATCH_MAX_QUBITS], rs[2][BATCH_MAX_QUBITS];
+  __shared__ RotFactor rf[2][BATCH_MAX_QUBITS];
   __shared__ Shared sh;
   const uint32_t b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int j = wmap[b];
@@ -468,19 +515,19 @@ __global__ __launch_bounds__(B) void k_wide_pass(const BInst* __restrict__ insts
   cx.t = b - (uint32_t)wtile0[j];
   const int own_lo = cx.tb ? cx.lowb : 0;
   float2* __restrict__ g = states + wstate[j];
-  // rotation tables: [0] layer pass-1, [1] layer pass (half angles)
+  // rotation tables: [0] layer pass-1, [1] layer pass (factored half angles)
   if (threadIdx.x < 2 * BATCH_MAX_QUBITS) {
     const int sel = threadIdx.x / BATCH_MAX_QUBITS, q = threadIdx.x % BATCH_MAX_QUBITS;
     const int layer = pass - 1 + sel;
-    float c = 1.0f, sn = 0.0f;
+    double c = 1.0, sn = 0.0;
     if (q < in.n && layer >= 0 && layer < L) {
-      if (qaoa) half_angle(param_of(params, in, it, L + layer), &c, &sn);
-      else half_angle(param_of(params, in, it, layer * in.n + q) / 2.0, &c, &sn);
+      const double h = qaoa ? param_of(params, in, it, L + layer) : param_of(params, in, it, layer * in.n + q) / 2.0;
+      sincos(h, &sn, &c);
     }
-    rc[sel][q] = c;
-    rs[sel][q] = sn;
+    rf[sel][q] = rot_factor(c, sn);
   }
   __syncthreads();
+  float2 scale = make_float2(1.0f, 0.0f);  // product of the factored-out rotation factors
   const int nw = (TQ - own_lo + R - 1) / R;  // windows covering the own bits, top first
   float2 v[NR];
   int lo;
@@ -506,7 +553,7 @@ __global__ __launch_bounds__(B) void k_wide_pass(const BInst* __restrict__ insts
         wide_transpose<TQ>(v, lo, wl, lds);
         lo = wl;
       }
-      wide_rotate<TQ>(v, lo, own_lo, dm, cx, qaoa, rc[0], rs[0]);
+      wide_rotate<TQ>(v, lo, own_lo, dm, cx, qaoa, rf[0], scale);
     }
     if (!qaoa) {
 #pragma unroll
@@ -529,7 +576,7 @@ __global__ __launch_bounds__(B) void k_wide_pass(const BInst* __restrict__ insts
         wide_transpose<TQ>(v, lo, wl, lds);
         lo = wl;
       }
-      wide_rotate<TQ>(v, lo, own_lo, dm, cx, qaoa, rc[1], rs[1]);
+      wide_rotate<TQ>(v, lo, own_lo, dm, cx, qaoa, rf[1], scale);
     }
   }
   if (!l2 && qaoa) {  // last QAOA pass: energy partial and argmax of the tile
@@ -542,6 +589,10 @@ __global__ __launch_bounds__(B) void k_wide_pass(const BInst* __restrict__ insts
       acc += ((double)a.x * a.x + (double)a.y * a.y) * (double)ct[gi];
       better(bv, bi, (double)(a.x * a.x + a.y * a.y), gi);
     }
+    // the pass's rotation factors scale every probability by |scale|^2
+    const double s2 = (double)scale.x * scale.x + (double)scale.y * scale.y;
+    acc *= s2;
+    bv *= s2;
     const double e = block_sum_all(acc, sh.red);
     block_argmax(bv, bi, sh.av, sh.ai);
     const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
@@ -549,7 +600,11 @@ __global__ __launch_bounds__(B) void k_wide_pass(const BInst* __restrict__ insts
     return;
   }
 #pragma unroll
-  for (int m = 0; m < NR; ++m) g[cx.gidx(win_index<R>(threadIdx.x, lo, m))] = v[m];
+  for (int m = 0; m < NR; ++m) {
+    const float2 a = v[m];
+    g[cx.gidx(win_index<R>(threadIdx.x, lo, m))] =
+        make_float2(scale.x * a.x - scale.y * a.y, scale.x * a.y + scale.y * a.x);
+  }
 }
 
 template __global__ void k_wide_pass<9>(const BInst*, const BItem*, const int32_t*, const int32_t*, const int64_t*,

@@ -112,3 +112,24 @@ def test_batch_wide_edge_tilings_on_cpu():
         assert abs(en[k] - want) <= 1e-5 * (1 + sum(w)), (n, en[k], want)
         pr = np.abs(amps) ** 2
         assert pr[am[k]] >= pr.max() * (1 - 1e-4)
+
+
+def test_wide_rotation_factoring_modes():
+    """Wide states rotate with the larger of cos / sin factored out of each
+    2x2 (4 forms: Rx or Ry, cos or sin factored) and one uniform scale per
+    pass; angles on both sides of pi/4 exercise all four, over several
+    passes."""
+    b = Batch()
+    e, w = workloads.erdos_renyi_graph(14, 0.5, 77)
+    g, be = [0.4, 1.1, 2.9], [0.3, 1.4, 2.6]  # beta: cos-dominant, sin-dominant, cos-dominant (negative)
+    b.add_qaoa(14, e, w, 3, g, be)
+    terms = workloads.random_pauli_hamiltonian(14, 10, 3, 78)
+    prm = list(np.random.default_rng(79).uniform(-math.pi, math.pi, 14 * 2))  # half angles up to pi/2
+    b.add_vqe(14, 2, terms, prm)
+    en, am = batch_emu.evaluate(b, grid=2)
+    want, amps = orc.qaoa_cost(14, e, w, g, be, want_state=True)
+    assert abs(en[0] - want) <= 1e-5 * (1 + sum(w)), (en[0], want)
+    pr = np.abs(amps) ** 2
+    assert pr[am[0]] >= pr.max() * (1 - 1e-4)
+    want_v = orc.energy(14, workloads.ry_cz_ansatz(14, 2, prm).to_array(), terms)
+    assert abs(en[1] - want_v) <= 1e-5 * (1 + sum(abs(c) for c, _ in terms)), (en[1], want_v)
