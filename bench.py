@@ -13,6 +13,9 @@ circuit's original gate applications per second.  The line also carries:
   cpu_baseline  the reference's f32 CPU path (QuantumStateF32 scalar loops,
                 restated in the oracle), one host thread, on the first gates
                 of the same circuit;
+  cpu_baseline_simd  the reference's fastest CPU path beside it: simd.rs's
+                AVX2 + thread-pool kernels on a complex-f64 state (restated
+                in the oracle), up to 16 host threads, same leading gates;
   qaoa          the batched QAOA part of the metric: held-out MaxCut
                 instances (G(n, 0.5), n ~ U{10..22}, U(0,1) weights,
                 p ~ U{1..8}) through the plain batched optimizer (run_qaoa's
@@ -66,6 +69,31 @@ def cpu_baseline(nq, arr, budget_s=20.0):
     return {"value": done / el, "unit": "gate-apps/s", "cores": 1, "kind": "port",
             "sample": "first %d gates of the %d-qubit c32 circuit on the restated QuantumStateF32 scalar path "
                       "(mixed_precision.rs:396-458), %.1f s" % (done, nq, el)}
+
+
+def cpu_simd_baseline(nq, arr, budget_s=12.0):
+    """Oracle restatement of simd.rs's *_best kernels (simd.rs:270-377: AVX2
+    with a thread pool over a complex-f64 state, the reference's fastest CPU
+    path), on the leading gates of the same circuit, with the host threads
+    this process may use (at most 16)."""
+    from oracle import oracle as orc
+    threads = max(1, min(len(os.sched_getaffinity(0)), 16))
+    amps = np.zeros(1 << nq, dtype=np.complex128)
+    amps[0] = 1
+    two = (orc.CNOT, orc.CZ, orc.SWAP, orc.RZZ)
+    done = 0
+    t0 = time.perf_counter()
+    while done < len(arr) and time.perf_counter() - t0 < budget_s:
+        g = np.ascontiguousarray(arr[done:done + 1]).view(orc.GATE_DTYPE)[0]
+        if int(g["kind"]) in two:
+            orc.simd_apply_2q_best(amps, nq, int(g["q0"]), int(g["q1"]), orc.matrix_2q(g), threads)
+        else:
+            orc.simd_apply_1q_best(amps, nq, int(g["q0"]), orc.matrix_1q(g), threads)
+        done += 1
+    el = time.perf_counter() - t0
+    return {"value": done / el, "unit": "gate-apps/s", "cores": threads, "kind": "port",
+            "sample": "first %d gates of the %d-qubit circuit on the restated simd.rs *_best path (complex f64, "
+                      "AVX2, %d threads), %.1f s" % (done, nq, threads, el)}
 
 
 def measured_traffic(path):
@@ -211,10 +239,12 @@ def main():
                          "the fused tile schedule above needs %d passes instead of %d" % (passes, blocks)}
         log("dense: %s" % dense)
 
-    cpu = None
+    cpu = cpu_simd = None
     if not args.no_cpu:
         log("cpu baseline ...")
         cpu = cpu_baseline(nq, arr, args.cpu_budget)
+        log("cpu simd baseline ...")
+        cpu_simd = cpu_simd_baseline(nq, arr, min(args.cpu_budget, 12.0))
 
     qaoa = None
     params_path = os.path.join(ROOT, "ruqu_amd", "data", "warmstart_params.npy")
@@ -261,6 +291,7 @@ def main():
         "per_gate_1q_GBps": per_gate,
         "copy_GBps": copy_gbs,
         "cpu_baseline": cpu,
+        "cpu_baseline_simd": cpu_simd,
         "f32_error_vs_f64": err,
         "dense_blocks": dense,
         "qaoa": qaoa,
