@@ -332,7 +332,7 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--eval", type=int, default=4096)
-    ap.add_argument("--out", default="gpurun_out/warmstart")
+    ap.add_argument("--out", default="warmstart_out")
     ap.add_argument("--seed", type=int, default=0)
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)

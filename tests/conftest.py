@@ -12,7 +12,7 @@ def pytest_configure(config):
 
 def _ensure_built():
     """Build the oracle and the product library in-tree when they are missing
-    (this container has gcc and hipcc; the GPU box receives prebuilt files)."""
+    (needs gcc and hipcc; a machine without them uses the prebuilt files)."""
     import subprocess
     oracle_so = os.path.join(ROOT, "oracle", "liboracle.so")
     if not os.path.exists(oracle_so):
