@@ -362,6 +362,12 @@ int rq_batch_plan_export(rq_batch* b, int round, int32_t* info, int64_t* info64,
 /* Host-only: instance (48 B), edge (16 B), Pauli-term (32 B) and parameter
  * records in device layout; counts[4] = {instances, edges, terms, params}. */
 int rq_batch_data_export(rq_batch* b, void* insts, void* edges, void* terms, double* params, size_t* counts);
+/* Host-only: the pass structure of a round's wide items (states above 13
+ * qubits): first / last pass, the state the first pass reads (a shared prefix
+ * state when the first pass is > 0), and every pass's tile list
+ * (concatenated; pass s at pass_off[s]). */
+int rq_batch_plan_passes(rq_batch* b, int round, int32_t* wfirst, int32_t* wlast, int64_t* wsrc, int32_t* ptile,
+                         int32_t* pass_off);
 
 /* ---- fused schedule (no reference counterpart: the reference's optimizer
  * only folds adjacent 1q gates, optimizer.rs:46-94, and is not called by

@@ -14,6 +14,14 @@
 // circuits of QAOA parameter r-1.  Each round is one launch of the persistent
 // kernel; the update kernels run on the device between rounds, so a whole
 // iteration is captured once as a hipGraph and replayed.
+//
+// Shared prefixes (states above 13 qubits): the two circuits shifted at
+// gamma_l or beta_l agree with each other, at the angles updated so far, up to
+// the input of pass l (pass s applies layer s's cost phase and starts its
+// mixer).  Each such instance keeps a running prefix state; in the round of
+// gamma_l (beta_l) an advance item runs pass l-1 on it with the current
+// angles, and the two shifted circuits start at pass l from it.  The gamma
+// rounds and the beta rounds each start the chain from |+> (l = 0).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -30,8 +38,8 @@ __global__ void k_batch_eval(const BInst* insts, const BItem* items, int nitems,
                              const float* ctab, double* energy, uint32_t* amax);
 template <int TQ>
 __global__ void k_wide_pass(const BInst* insts, const BItem* witems, const int32_t* wmap, const int32_t* wtile0,
-                            const int64_t* wstate, const int32_t* done, const double* params, const float* ctab,
-                            float2* states, int pass, BPart* parts);
+                            const int64_t* wstate, const WidePasses wp, const int32_t* done, const double* params,
+                            const float* ctab, float2* states, int pass, BPart* parts);
 __global__ void k_wide_energy(const BInst* insts, const BItem* witems, const int32_t* wmap, const int32_t* wtile0,
                               const int64_t* wstate, const int32_t* vmap, const int32_t* done, const BTerm* terms,
                               const float2* states, int tq, BPart* parts);
@@ -140,11 +148,21 @@ int upload(T** d, const std::vector<T>& h) {
 // layer's launch covers exactly their tiles.
 struct RoundPlan {
   int small_off = 0, nsmall = 0;  // into all_small
-  int wide_off = 0, nwide = 0;    // into all_wide / all_wtile0 / all_wstate
+  int wide_off = 0, nwide = 0;    // into all_wide / all_wtile0 / all_wstate / all_wfirst / all_wlast / all_wsrc
   int tile_off = 0, ntiles = 0;   // into all_wmap
-  std::vector<int> layer_grid;    // pass s: tiles of the items with more than s passes (p + 1)
+  std::vector<int> layer_grid;    // pass s: number of tiles of the items whose passes include s
+  std::vector<int> pass_off;      // pass s: its tile list in all_ptile
   int vmap_off = 0, nvtiles = 0;  // into all_vmap: the round's VQE tiles
-  int64_t state_amps = 0;         // HBM amplitudes of this round's wide states
+  int64_t state_amps = 0;         // HBM amplitudes needed (prefix states included)
+};
+
+// Passes of one wide item: first..last; the first pass (when > 0) reads the
+// state at src instead of the item's own slot.  prefix: the item is the
+// instance's advance item (its own slot is the prefix state).
+struct ItemPass {
+  int first = 0, last = -1;
+  int64_t src = -1;
+  bool prefix = false;
 };
 
 struct Batch {
@@ -170,14 +188,20 @@ struct Batch {
   std::vector<RoundPlan> rounds;
   RoundPlan base;
   std::vector<BItem> all_small, all_wide;
-  std::vector<int32_t> all_wtile0, all_wmap, all_vmap;
-  std::vector<int64_t> all_wstate;
+  std::vector<int32_t> all_wtile0, all_wmap, all_vmap, all_wfirst, all_wlast, all_ptile;
+  std::vector<int64_t> all_wstate, all_wsrc;
+  std::vector<int64_t> pfx_off;  // per instance: its prefix state (amplitudes), -1: none
+  int64_t prefix_amps = 0;       // prefix states, at the start of the state buffer
   BItem* d_small = nullptr;
   BItem* d_wide = nullptr;
   int32_t* d_wtile0 = nullptr;
   int32_t* d_wmap = nullptr;
   int32_t* d_vmap = nullptr;
   int64_t* d_wstate = nullptr;
+  int32_t* d_wfirst = nullptr;
+  int32_t* d_wlast = nullptr;
+  int64_t* d_wsrc = nullptr;
+  int32_t* d_ptile = nullptr;
   float2* d_states = nullptr;
   int64_t states_cap = 0;  // amplitudes
   BPart* d_parts = nullptr;
@@ -208,16 +232,30 @@ double item_cost(const BInst& in) {
   return in.kind == BK_QAOA ? amps * (in.p + 1) : amps * (in.p + in.nterms);
 }
 
-// Appends one round's items to the flat plan arrays.
-RoundPlan plan_round(Batch& b, std::vector<BItem> v) {
+// Appends one round's items to the flat plan arrays.  pv[k]: the passes of
+// item v[k] when it is wide (last < 0: all of them, 0..p).  Rounds after
+// round 0 keep their own states clear of the prefix states (which live from
+// one round to the next); round 0 and plain evaluation may overlap them (the
+// gamma chain restarts from |+> in round 1).
+RoundPlan plan_round(Batch& b, const std::vector<BItem>& v, std::vector<ItemPass> pv = {}, bool after_prefix = false) {
   RoundPlan rp;
-  std::vector<BItem> small, wide;
-  for (const BItem& it : v) (b.insts[it.inst].n <= BATCH_LDS_QUBITS ? small : wide).push_back(it);
+  pv.resize(v.size());
+  std::vector<BItem> small;
+  std::vector<std::pair<BItem, ItemPass>> wide;
+  for (size_t k = 0; k < v.size(); ++k) {
+    if (b.insts[v[k].inst].n <= BATCH_LDS_QUBITS) {
+      small.push_back(v[k]);
+      continue;
+    }
+    ItemPass ip = pv[k];
+    if (ip.last < 0) ip.last = b.insts[v[k].inst].p;
+    wide.emplace_back(v[k], ip);
+  }
   std::stable_sort(small.begin(), small.end(), [&](const BItem& x, const BItem& y) {
     return item_cost(b.insts[x.inst]) > item_cost(b.insts[y.inst]);
   });
-  std::stable_sort(wide.begin(), wide.end(), [&](const BItem& x, const BItem& y) {
-    const BInst &a = b.insts[x.inst], &c = b.insts[y.inst];
+  std::stable_sort(wide.begin(), wide.end(), [&](const auto& x, const auto& y) {
+    const BInst &a = b.insts[x.first.inst], &c = b.insts[y.first.inst];
     return a.p != c.p ? a.p > c.p : a.n > c.n;
   });
   rp.small_off = (int)b.all_small.size();
@@ -226,25 +264,43 @@ RoundPlan plan_round(Batch& b, std::vector<BItem> v) {
   rp.wide_off = (int)b.all_wide.size();
   rp.nwide = (int)wide.size();
   rp.tile_off = (int)b.all_wmap.size();
-  int tiles = 0;
-  int64_t amps = 0;
-  int maxl = 0;
-  for (int j = 0; j < rp.nwide; ++j) maxl = std::max(maxl, b.insts[wide[j].inst].p);
-  rp.layer_grid.assign(rp.nwide ? maxl + 1 : 0, 0);
+  int tiles = 0, maxpass = -1;
+  int64_t amps = after_prefix ? b.prefix_amps : 0;
   rp.vmap_off = (int)b.all_vmap.size();
   for (int j = 0; j < rp.nwide; ++j) {
-    const BInst& in = b.insts[wide[j].inst];
+    const BInst& in = b.insts[wide[j].first.inst];
+    const ItemPass& ip = wide[j].second;
     const int nt = 1 << (in.n - b.tq);
-    b.all_wide.push_back(wide[j]);
+    b.all_wide.push_back(wide[j].first);
     b.all_wtile0.push_back(tiles);
-    b.all_wstate.push_back(amps);
+    b.all_wfirst.push_back(ip.first);
+    b.all_wlast.push_back(ip.last);
+    if (ip.prefix) {
+      b.all_wstate.push_back(b.pfx_off[wide[j].first.inst]);
+    } else {
+      b.all_wstate.push_back(amps);
+      amps += (int64_t)1 << in.n;
+    }
+    b.all_wsrc.push_back(ip.first > 0 ? ip.src : b.all_wstate.back());
     for (int t = 0; t < nt; ++t) {
       b.all_wmap.push_back(j);
       if (in.kind == BK_VQE) b.all_vmap.push_back(tiles + t);
     }
     tiles += nt;
-    amps += (int64_t)1 << in.n;
-    for (int l = 0; l <= in.p; ++l) rp.layer_grid[l] = tiles;  // sorted by p: a prefix
+    maxpass = std::max(maxpass, ip.last);
+  }
+  // pass s: the tiles of every item whose passes include s, item by item
+  rp.layer_grid.assign(maxpass + 1, 0);
+  rp.pass_off.assign(maxpass + 1, 0);
+  for (int s = 0; s <= maxpass; ++s) {
+    rp.pass_off[s] = (int)b.all_ptile.size();
+    for (int j = 0; j < rp.nwide; ++j) {
+      const ItemPass& ip = wide[j].second;
+      if (s < ip.first || s > ip.last) continue;
+      const int t0 = b.all_wtile0[rp.wide_off + j], nt = 1 << (b.insts[wide[j].first.inst].n - b.tq);
+      for (int t = 0; t < nt; ++t) b.all_ptile.push_back(t0 + t);
+    }
+    rp.layer_grid[s] = (int)b.all_ptile.size() - rp.pass_off[s];
   }
   rp.nvtiles = (int)b.all_vmap.size() - rp.vmap_off;
   rp.ntiles = tiles;
@@ -273,7 +329,18 @@ void build_plan(Batch& b) {
   for (const BInst& in : b.insts)
     if (in.kind == BK_QAOA) pmax2 = std::max(pmax2, in.nparams);
   b.nrounds = 1 + pmax2;
+  // prefix states of the wide QAOA instances with a shift beyond layer 0
+  b.pfx_off.assign(ninst, -1);
+  b.prefix_amps = 0;
+  for (int k = 0; k < ninst; ++k) {
+    const BInst& in = b.insts[k];
+    if (in.kind == BK_QAOA && in.n > BATCH_LDS_QUBITS && in.p >= 2) {
+      b.pfx_off[k] = b.prefix_amps;
+      b.prefix_amps += (int64_t)1 << in.n;
+    }
+  }
   std::vector<std::vector<BItem>> rounds(b.nrounds);
+  std::vector<std::vector<ItemPass>> rpass(b.nrounds);
   std::vector<BItem> base;
   b.out0.assign(ninst, -1);
   b.outr.assign(ninst, -1);
@@ -294,8 +361,28 @@ void build_plan(Batch& b) {
     if (in.kind != BK_QAOA) continue;
     b.outr[k] = out;
     for (int j = 0; j < in.nparams; ++j) {
-      rounds[1 + j].push_back(BItem{k, 1, j, out++});
-      rounds[1 + j].push_back(BItem{k, 2, j, out++});
+      const int l = j < in.p ? j : j - in.p;  // gamma_l, then beta_l
+      const bool shared = b.pfx_off[k] >= 0 && l >= 1;
+      std::vector<BItem>& rv = rounds[1 + j];
+      std::vector<ItemPass>& pv = rpass[1 + j];
+      pv.resize(rv.size());
+      if (shared) {  // advance the prefix: pass l-1 at the current angles
+        ItemPass adv;
+        adv.first = adv.last = l - 1;
+        adv.src = b.pfx_off[k];
+        adv.prefix = true;
+        rv.push_back(BItem{k, 0, -1, -1});
+        pv.push_back(adv);
+      }
+      ItemPass sh;
+      if (shared) {
+        sh.first = l;
+        sh.src = b.pfx_off[k];
+      }
+      rv.push_back(BItem{k, 1, j, out++});
+      pv.push_back(sh);
+      rv.push_back(BItem{k, 2, j, out++});
+      pv.push_back(sh);
     }
   }
   b.nouts = out;
@@ -305,8 +392,12 @@ void build_plan(Batch& b) {
   b.all_wmap.clear();
   b.all_vmap.clear();
   b.all_wstate.clear();
+  b.all_wfirst.clear();
+  b.all_wlast.clear();
+  b.all_wsrc.clear();
+  b.all_ptile.clear();
   b.rounds.clear();
-  for (int r = 0; r < b.nrounds; ++r) b.rounds.push_back(plan_round(b, rounds[r]));
+  for (int r = 0; r < b.nrounds; ++r) b.rounds.push_back(plan_round(b, rounds[r], rpass[r], r > 0));
   b.base = plan_round(b, base);
   b.planned = true;
   b.plan_on_device = false;
@@ -326,12 +417,20 @@ int upload_plan(Batch& b) {
   free_dev(b.d_wmap);
   free_dev(b.d_vmap);
   free_dev(b.d_wstate);
+  free_dev(b.d_wfirst);
+  free_dev(b.d_wlast);
+  free_dev(b.d_wsrc);
+  free_dev(b.d_ptile);
   int rc = upload(&b.d_small, b.all_small);
   if (!rc) rc = upload(&b.d_wide, b.all_wide);
   if (!rc) rc = upload(&b.d_wtile0, b.all_wtile0);
   if (!rc) rc = upload(&b.d_wmap, b.all_wmap);
   if (!rc) rc = upload(&b.d_vmap, b.all_vmap);
   if (!rc) rc = upload(&b.d_wstate, b.all_wstate);
+  if (!rc) rc = upload(&b.d_wfirst, b.all_wfirst);
+  if (!rc) rc = upload(&b.d_wlast, b.all_wlast);
+  if (!rc) rc = upload(&b.d_wsrc, b.all_wsrc);
+  if (!rc) rc = upload(&b.d_ptile, b.all_ptile);
   if (rc) return rc;
   int64_t amps = b.base.state_amps;
   int parts = b.base.ntiles;
@@ -432,17 +531,20 @@ void enqueue_round(Batch& b, const RoundPlan& rp, unsigned* counter, const int32
   const int64_t* ws = b.d_wstate + rp.wide_off;
   const size_t lds = wide_lds_bytes(b.tq);
   for (int ps = 0; ps < (int)rp.layer_grid.size(); ++ps) {
+    if (!rp.layer_grid[ps]) continue;
     const dim3 g(rp.layer_grid[ps]), t(BATCH_THREADS);
+    const WidePasses wp{b.d_ptile + rp.pass_off[ps], b.d_wfirst + rp.wide_off, b.d_wlast + rp.wide_off,
+                        b.d_wsrc + rp.wide_off};
     switch (b.tq) {
-      case 9: hipLaunchKernelGGL(k_wide_pass<9>, g, t, lds, b.st, b.d_insts, wi, wm, w0, ws, done, b.d_params,
+      case 9: hipLaunchKernelGGL(k_wide_pass<9>, g, t, lds, b.st, b.d_insts, wi, wm, w0, ws, wp, done, b.d_params,
                                  b.d_ctab, b.d_states, ps, b.d_parts); break;
-      case 10: hipLaunchKernelGGL(k_wide_pass<10>, g, t, lds, b.st, b.d_insts, wi, wm, w0, ws, done, b.d_params,
+      case 10: hipLaunchKernelGGL(k_wide_pass<10>, g, t, lds, b.st, b.d_insts, wi, wm, w0, ws, wp, done, b.d_params,
                                   b.d_ctab, b.d_states, ps, b.d_parts); break;
-      case 11: hipLaunchKernelGGL(k_wide_pass<11>, g, t, lds, b.st, b.d_insts, wi, wm, w0, ws, done, b.d_params,
+      case 11: hipLaunchKernelGGL(k_wide_pass<11>, g, t, lds, b.st, b.d_insts, wi, wm, w0, ws, wp, done, b.d_params,
                                   b.d_ctab, b.d_states, ps, b.d_parts); break;
-      case 12: hipLaunchKernelGGL(k_wide_pass<12>, g, t, lds, b.st, b.d_insts, wi, wm, w0, ws, done, b.d_params,
+      case 12: hipLaunchKernelGGL(k_wide_pass<12>, g, t, lds, b.st, b.d_insts, wi, wm, w0, ws, wp, done, b.d_params,
                                   b.d_ctab, b.d_states, ps, b.d_parts); break;
-      default: hipLaunchKernelGGL(k_wide_pass<13>, g, t, lds, b.st, b.d_insts, wi, wm, w0, ws, done, b.d_params,
+      default: hipLaunchKernelGGL(k_wide_pass<13>, g, t, lds, b.st, b.d_insts, wi, wm, w0, ws, wp, done, b.d_params,
                                   b.d_ctab, b.d_states, ps, b.d_parts); break;
     }
   }
@@ -475,7 +577,8 @@ void free_batch(Batch& b) {
   void* ptrs[] = {b.d_insts,  b.d_edges,  b.d_terms,  b.d_params, b.d_ctab,     b.d_small,   b.d_wide,
                   b.d_wtile0, b.d_wmap,   b.d_vmap,   b.d_wstate, b.d_states, b.d_parts,   b.d_energy,  b.d_amax,
                   b.d_counters, b.d_out0, b.d_outr,   b.o.iter,   b.o.hlen,     b.o.conv,    b.o.done,
-                  b.o.hist,   b.o.best,   b.o.best_idx, b.d_best_params, b.d_zero_done, b.d_cutchunks};
+                  b.o.hist,   b.o.best,   b.o.best_idx, b.d_best_params, b.d_zero_done, b.d_cutchunks,
+                  b.d_wfirst, b.d_wlast,  b.d_wsrc,   b.d_ptile};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (b.st) (void)hipStreamDestroy(b.st);
@@ -763,6 +866,28 @@ int rq_batch_plan_export(rq_batch* h, int round, int32_t* info, int64_t* info64,
   if (wmap) std::memcpy(wmap, b->all_wmap.data() + rp.tile_off, rp.ntiles * sizeof(int32_t));
   if (pass_grid) std::copy(rp.layer_grid.begin(), rp.layer_grid.end(), pass_grid);
   if (vmap) std::memcpy(vmap, b->all_vmap.data() + rp.vmap_off, rp.nvtiles * sizeof(int32_t));
+  return RQ_OK;
+}
+
+// Host-only: the pass structure of one round's wide items: first / last pass
+// and first-pass source state per item, and the tile list of every pass
+// (concatenated; pass s starts at pass_off[s]).
+int rq_batch_plan_passes(rq_batch* h, int round, int32_t* wfirst, int32_t* wlast, int64_t* wsrc, int32_t* ptile,
+                         int32_t* pass_off) {
+  Batch* b = (Batch*)h;
+  if (!b || b->insts.empty()) return fail(RQ_ERR_ARGUMENT, "empty batch");
+  if (!b->planned) build_plan(*b);
+  if (round >= b->nrounds) return fail(RQ_ERR_ARGUMENT, "round out of range");
+  const RoundPlan& rp = round < 0 ? b->base : b->rounds[round];
+  if (wfirst) std::memcpy(wfirst, b->all_wfirst.data() + rp.wide_off, rp.nwide * sizeof(int32_t));
+  if (wlast) std::memcpy(wlast, b->all_wlast.data() + rp.wide_off, rp.nwide * sizeof(int32_t));
+  if (wsrc) std::memcpy(wsrc, b->all_wsrc.data() + rp.wide_off, rp.nwide * sizeof(int64_t));
+  int64_t total = 0;
+  for (int g : rp.layer_grid) total += g;
+  if (ptile && total)
+    std::memcpy(ptile, b->all_ptile.data() + (rp.pass_off.empty() ? 0 : rp.pass_off[0]), total * sizeof(int32_t));
+  if (pass_off)
+    for (size_t s = 0; s < rp.pass_off.size(); ++s) pass_off[s] = rp.pass_off[s] - rp.pass_off[0];
   return RQ_OK;
 }
 

@@ -36,6 +36,16 @@ struct BEdge {
   double w;
 };
 
+// Per-pass launch data of k_wide_pass: the pass's tiles (round tile ids, item
+// by item) and every item's first / last pass and the state its first pass
+// reads (amplitude offset; a shared prefix when the first pass is > 0).
+struct WidePasses {
+  const int32_t* ptile;
+  const int32_t* first;
+  const int32_t* last;
+  const int64_t* src;
+};
+
 // One workgroup of k_cut_tables: entries [chunk * BATCH_CUT_CHUNK, ...) of
 // instance inst's cut table.
 struct BCutChunk {

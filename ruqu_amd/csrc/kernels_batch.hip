@@ -492,7 +492,7 @@ template <int TQ>
 __global__ __launch_bounds__(B) void k_wide_pass(const BInst* __restrict__ insts, const BItem* __restrict__ witems,
                                                  const int32_t* __restrict__ wmap,
                                                  const int32_t* __restrict__ wtile0,
-                                                 const int64_t* __restrict__ wstate,
+                                                 const int64_t* __restrict__ wstate, const WidePasses wp,
                                                  const int32_t* __restrict__ done_flags,
                                                  const double* __restrict__ params, const float* __restrict__ ctab,
                                                  float2* __restrict__ states, int pass, BPart* __restrict__ parts) {
@@ -500,7 +500,8 @@ __global__ __launch_bounds__(B) void k_wide_pass(const BInst* __restrict__ insts
   extern __shared__ float2 lds[];  // 2^TQ + 2^(TQ-R) slots
   __shared__ RotFactor rf[2][BATCH_MAX_QUBITS];
   __shared__ Shared sh;
-  const uint32_t b = xcd_swizzle(blockIdx.x, gridDim.x);
+  // this pass's tile list (round tile ids); item j's first pass reads wp.src
+  const uint32_t b = (uint32_t)wp.ptile[xcd_swizzle(blockIdx.x, gridDim.x)];
   const int j = wmap[b];
   const BItem it = witems[j];
   const BInst in = insts[it.inst];
@@ -541,8 +542,9 @@ __global__ __launch_bounds__(B) void k_wide_pass(const BInst* __restrict__ insts
     }
   } else {
     lo = TQ - R;
+    const float2* __restrict__ src = pass == wp.first[j] ? states + wp.src[j] : g;
 #pragma unroll
-    for (int m = 0; m < NR; ++m) v[m] = g[cx.gidx(win_index<R>(threadIdx.x, lo, m))];
+    for (int m = 0; m < NR; ++m) v[m] = src[cx.gidx(win_index<R>(threadIdx.x, lo, m))];
   }
   const float* __restrict__ ct = qaoa ? ctab + in.ctab_off : nullptr;
   if (l1) {  // complete layer pass-1: windows top -> bottom
@@ -608,15 +610,15 @@ __global__ __launch_bounds__(B) void k_wide_pass(const BInst* __restrict__ insts
 }
 
 template __global__ void k_wide_pass<9>(const BInst*, const BItem*, const int32_t*, const int32_t*, const int64_t*,
-                                        const int32_t*, const double*, const float*, float2*, int, BPart*);
+                                        const WidePasses, const int32_t*, const double*, const float*, float2*, int, BPart*);
 template __global__ void k_wide_pass<10>(const BInst*, const BItem*, const int32_t*, const int32_t*, const int64_t*,
-                                         const int32_t*, const double*, const float*, float2*, int, BPart*);
+                                         const WidePasses, const int32_t*, const double*, const float*, float2*, int, BPart*);
 template __global__ void k_wide_pass<11>(const BInst*, const BItem*, const int32_t*, const int32_t*, const int64_t*,
-                                         const int32_t*, const double*, const float*, float2*, int, BPart*);
+                                         const WidePasses, const int32_t*, const double*, const float*, float2*, int, BPart*);
 template __global__ void k_wide_pass<12>(const BInst*, const BItem*, const int32_t*, const int32_t*, const int64_t*,
-                                         const int32_t*, const double*, const float*, float2*, int, BPart*);
+                                         const WidePasses, const int32_t*, const double*, const float*, float2*, int, BPart*);
 template __global__ void k_wide_pass<13>(const BInst*, const BItem*, const int32_t*, const int32_t*, const int64_t*,
-                                         const int32_t*, const double*, const float*, float2*, int, BPart*);
+                                         const WidePasses, const int32_t*, const double*, const float*, float2*, int, BPart*);
 
 // VQE tile partial sum_k c_k Re(conj(a_{i^x_k}) phase_k(i) a_i) over the
 // tile's indices i (natural order: tile t = indices t*2^tq ..).  Block b
@@ -663,6 +665,7 @@ __global__ __launch_bounds__(B) void k_wide_fold(const BInst* __restrict__ insts
   __shared__ Shared sh;
   const int j = blockIdx.x;
   const BItem it = witems[j];
+  if (it.out < 0) return;  // an advance item (shared prefix state): no energy
   const BInst in = insts[it.inst];
   const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
   if (done[it.inst]) {

@@ -103,14 +103,16 @@ int emu_batch_eval(const rq::BInst* insts, const rq::BItem* items, int nitems, c
 
 int emu_wide_pass(int grid, const rq::BInst* insts, const rq::BItem* wi, const int32_t* wmap, const int32_t* wtile0,
                   const int64_t* wstate, const int32_t* done, const double* params, const float* ctab, float2* states,
-                  int tq, int pass, rq::BPart* parts) {
+                  int tq, int pass, rq::BPart* parts, const int32_t* ptile, const int32_t* first, const int32_t* last,
+                  const int64_t* src) {
+  const rq::WidePasses wp{ptile, first, last, src};
   run_grid(grid, [&] {
     switch (tq) {
-      case 9: rq::k_wide_pass<9>(insts, wi, wmap, wtile0, wstate, done, params, ctab, states, pass, parts); break;
-      case 10: rq::k_wide_pass<10>(insts, wi, wmap, wtile0, wstate, done, params, ctab, states, pass, parts); break;
-      case 11: rq::k_wide_pass<11>(insts, wi, wmap, wtile0, wstate, done, params, ctab, states, pass, parts); break;
-      case 12: rq::k_wide_pass<12>(insts, wi, wmap, wtile0, wstate, done, params, ctab, states, pass, parts); break;
-      default: rq::k_wide_pass<13>(insts, wi, wmap, wtile0, wstate, done, params, ctab, states, pass, parts); break;
+      case 9: rq::k_wide_pass<9>(insts, wi, wmap, wtile0, wstate, wp, done, params, ctab, states, pass, parts); break;
+      case 10: rq::k_wide_pass<10>(insts, wi, wmap, wtile0, wstate, wp, done, params, ctab, states, pass, parts); break;
+      case 11: rq::k_wide_pass<11>(insts, wi, wmap, wtile0, wstate, wp, done, params, ctab, states, pass, parts); break;
+      case 12: rq::k_wide_pass<12>(insts, wi, wmap, wtile0, wstate, wp, done, params, ctab, states, pass, parts); break;
+      default: rq::k_wide_pass<13>(insts, wi, wmap, wtile0, wstate, wp, done, params, ctab, states, pass, parts); break;
     }
   });
   return 0;

@@ -36,7 +36,7 @@ def _emu():
     vp, i = C.c_void_p, C.c_int
     lib.emu_cut_tables.argtypes = [vp, vp, vp, vp, i]
     lib.emu_batch_eval.argtypes = [vp, vp, i] + [vp] * 7 + [i]
-    lib.emu_wide_pass.argtypes = [i] + [vp] * 9 + [i, i, vp]
+    lib.emu_wide_pass.argtypes = [i] + [vp] * 9 + [i, i, vp] + [vp] * 4
     lib.emu_wide_energy.argtypes = [i] + [vp] * 9 + [i, vp]
     lib.emu_wide_fold.argtypes = [i] + [vp] * 5 + [i, vp, vp]
     return lib
@@ -76,7 +76,47 @@ def plan(batch, rnd=-1):
     r.update(nsmall=nsmall, nwide=nwide, ntiles=ntiles, npasses=npasses, nvtiles=nvtiles, nrounds=int(info[4]),
              nouts=int(info[5]), tq=int(info[6]), state_amps=int(info64[0]), ctab_total=int(info64[1]))
     r["pass_grid"] = r["pass_grid"][:npasses]
+    lib.rq_batch_plan_passes.argtypes = [C.c_void_p, C.c_int, I32P, I32P, I64P, I32P, I32P]
+    total = int(r["pass_grid"].sum()) if npasses else 0
+    r.update(wfirst=np.zeros(max(nwide, 1), dtype=np.int32), wlast=np.zeros(max(nwide, 1), dtype=np.int32),
+             wsrc=np.zeros(max(nwide, 1), dtype=np.int64), ptile=np.zeros(max(total, 1), dtype=np.int32),
+             pass_off=np.zeros(max(npasses, 1), dtype=np.int32))
+    assert lib.rq_batch_plan_passes(batch._h, rnd, ip(r["wfirst"]), ip(r["wlast"]), lp(r["wsrc"]), ip(r["ptile"]),
+                                    ip(r["pass_off"])) == 0
     return r
+
+
+def _cut_tables(lib, insts, ea, total):
+    ctab = np.zeros(max(total, 1), dtype=np.float32)
+    # the launch shape of batch.cpp: BATCH_CUT_CHUNK = 4096 entries per workgroup
+    chunks = np.array([(k, c) for k in range(len(insts)) if int(insts[k]["ctab_off"]) >= 0
+                       for c in range(-(-(1 << int(insts[k]["n"])) // 4096))],
+                      dtype=np.dtype([("inst", "<i4"), ("chunk", "<u4")]))
+    if len(chunks):
+        lib.emu_cut_tables(_p(ctab), _p(insts), _p(ea), _p(chunks), len(chunks))
+    return ctab
+
+
+def _run_round(lib, d, r, ctab, states, energy, amax, grid):
+    """One round's launches (batch.cpp's enqueue_round) on host threads."""
+    insts, ea, ta, pa = d["insts"], d["edges"], d["terms"], d["params"]
+    done = np.zeros(len(insts), dtype=np.int32)
+    if r["nsmall"]:
+        lib.emu_batch_eval(_p(insts), _p(r["small"]), r["nsmall"], _p(done), _p(ea), _p(ta), _p(pa), _p(ctab),
+                           _p(energy), _p(amax), min(grid, r["nsmall"]))
+    if r["nwide"]:
+        parts = np.zeros(r["ntiles"], dtype=PART)
+        w = (_p(insts), _p(r["wide"]), _p(r["wmap"]), _p(r["wtile0"]), _p(r["wstate"]), _p(done))
+        for ps, g in enumerate(r["pass_grid"]):
+            if g:
+                lib.emu_wide_pass(int(g), *w, _p(pa), _p(ctab), _p(states), r["tq"], ps, _p(parts),
+                                  _p(r["ptile"][int(r["pass_off"][ps]):]), _p(r["wfirst"]), _p(r["wlast"]),
+                                  _p(r["wsrc"]))
+        if r["nvtiles"]:
+            lib.emu_wide_energy(r["nvtiles"], *w[:5], _p(r["vmap"]), _p(done), _p(ta), _p(states), r["tq"],
+                                _p(parts))
+        lib.emu_wide_fold(r["nwide"], _p(insts), _p(r["wide"]), _p(r["wtile0"]), _p(done), _p(parts), r["tq"],
+                          _p(energy), _p(amax))
 
 
 def evaluate(batch, grid=3):
@@ -85,34 +125,33 @@ def evaluate(batch, grid=3):
     host threads over the exported plan."""
     d, r = data(batch), plan(batch, -1)
     lib = _emu()
-    insts, ea, ta, pa = d["insts"], d["edges"], d["terms"], d["params"]
-    ctab = np.zeros(max(r["ctab_total"], 1), dtype=np.float32)
-    # the launch shape of batch.cpp: BATCH_CUT_CHUNK = 4096 entries per workgroup
-    chunks = np.array([(k, c) for k in range(len(insts)) if int(insts[k]["ctab_off"]) >= 0
-                       for c in range(-(-(1 << int(insts[k]["n"])) // 4096))],
-                      dtype=np.dtype([("inst", "<i4"), ("chunk", "<u4")]))
-    if len(chunks):
-        lib.emu_cut_tables(_p(ctab), _p(insts), _p(ea), _p(chunks), len(chunks))
+    ctab = _cut_tables(lib, d["insts"], d["edges"], r["ctab_total"])
     energy = np.zeros(r["nouts"], dtype=np.float64)
     amax = np.zeros(r["nouts"], dtype=np.uint32)
-    done = np.zeros(len(insts), dtype=np.int32)
-    if r["nsmall"]:
-        lib.emu_batch_eval(_p(insts), _p(r["small"]), r["nsmall"], _p(done), _p(ea), _p(ta), _p(pa), _p(ctab),
-                           _p(energy), _p(amax), min(grid, r["nsmall"]))
-    if r["nwide"]:
-        states = np.zeros(max(r["state_amps"], 1), dtype=np.complex64)
-        parts = np.zeros(r["ntiles"], dtype=PART)
-        w = (_p(insts), _p(r["wide"]), _p(r["wmap"]), _p(r["wtile0"]), _p(r["wstate"]), _p(done))
-        for ps, g in enumerate(r["pass_grid"]):
-            lib.emu_wide_pass(int(g), *w, _p(pa), _p(ctab), _p(states), r["tq"], ps, _p(parts))
-        if r["nvtiles"]:
-            lib.emu_wide_energy(r["nvtiles"], *w[:5], _p(r["vmap"]), _p(done), _p(ta), _p(states), r["tq"],
-                                _p(parts))
-        lib.emu_wide_fold(r["nwide"], _p(insts), _p(r["wide"]), _p(r["wtile0"]), _p(done), _p(parts), r["tq"],
-                          _p(energy), _p(amax))
-    en = np.zeros(len(insts))
-    am = np.zeros(len(insts), dtype=np.uint64)
+    states = np.zeros(max(r["state_amps"], 1), dtype=np.complex64)
+    _run_round(lib, d, r, ctab, states, energy, amax, grid)
+    en = np.zeros(len(d["insts"]))
+    am = np.zeros(len(d["insts"]), dtype=np.uint64)
     for it in list(r["small"]) + list(r["wide"]):
-        en[it["inst"]] = energy[it["out"]]
-        am[it["inst"]] = amax[it["out"]]
+        if it["out"] >= 0:
+            en[it["inst"]] = energy[it["out"]]
+            am[it["inst"]] = amax[it["out"]]
     return en, am
+
+
+def evaluate_rounds(batch, rounds, grid=3):
+    """Rounds of one optimizer iteration, in the given order, at fixed
+    parameters (no updates between them) and over one state buffer, so that
+    the shared prefix states carry from round to round.  Returns the energy
+    of every output slot."""
+    d = data(batch)
+    lib = _emu()
+    plans = [plan(batch, rnd) for rnd in rounds]
+    ctab = _cut_tables(lib, d["insts"], d["edges"], plans[0]["ctab_total"])
+    nouts = plans[0]["nouts"]
+    energy = np.zeros(nouts, dtype=np.float64)
+    amax = np.zeros(nouts, dtype=np.uint32)
+    states = np.zeros(max(max(p["state_amps"] for p in plans), 1), dtype=np.complex64)
+    for r in plans:
+        _run_round(lib, d, r, ctab, states, energy, amax, grid)
+    return energy

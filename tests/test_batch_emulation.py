@@ -133,3 +133,41 @@ def test_wide_rotation_factoring_modes():
     assert pr[am[0]] >= pr.max() * (1 - 1e-4)
     want_v = orc.energy(14, workloads.ry_cz_ansatz(14, 2, prm).to_array(), terms)
     assert abs(en[1] - want_v) <= 1e-5 * (1 + sum(abs(c) for c, _ in terms)), (en[1], want_v)
+
+
+def test_shared_prefix_rounds_match_shifted_circuits():
+    """Rounds 1.. of an optimizer iteration at fixed angles: the circuits
+    shifted at gamma_l / beta_l start at pass l from the instance's running
+    prefix state (advanced by one pass per round), and each must give the
+    energy of the full circuit with that one angle shifted by +-pi/2."""
+    b = Batch()
+    qc = []
+    for k, (n, p) in enumerate([(14, 3), (15, 2), (14, 1), (9, 2)]):
+        e, w = workloads.erdos_renyi_graph(n, 0.5, 500 + k)
+        rng = np.random.default_rng(90 + k)
+        qc.append((n, e, w, p, list(rng.uniform(0, math.pi, p)), list(rng.uniform(0, math.pi, p))))
+        b.add_qaoa(*qc[-1])
+    b.add_vqe(14, 2, workloads.random_pauli_hamiltonian(14, 6, 3, 91))
+    r0 = batch_emu.plan(b, 0)
+    rounds = list(range(1, r0["nrounds"]))
+    energy = batch_emu.evaluate_rounds(b, rounds)
+    seen_shared = 0
+    for rnd in rounds:
+        r = batch_emu.plan(b, rnd)
+        items = list(r["small"]) + list(r["wide"])
+        first = dict(zip([(int(it["inst"]), int(it["variant"])) for it in r["wide"]], r["wfirst"][:r["nwide"]]))
+        for it in items:
+            if it["out"] < 0:
+                continue
+            k, j, var = int(it["inst"]), int(it["param"]), int(it["variant"])
+            n, e, w, p, g, be = qc[k]
+            g2, b2 = list(g), list(be)
+            shift = math.pi / 2 if var == 1 else -math.pi / 2
+            if j < p:
+                g2[j] += shift
+            else:
+                b2[j - p] += shift
+            want = orc.qaoa_cost(n, e, w, g2, b2)
+            assert abs(energy[it["out"]] - want) <= 1e-5 * (1 + sum(w)), (rnd, k, j, var, energy[it["out"]], want)
+            seen_shared += first.get((k, var), 0) > 0
+    assert seen_shared > 0  # some shifted circuits did start from a shared prefix
