@@ -157,12 +157,12 @@ struct RoundPlan {
 };
 
 // Passes of one wide item: first..last; the first pass (when > 0) reads the
-// state at src instead of the item's own slot.  prefix: the item is the
-// instance's advance item (its own slot is the prefix state).
+// state at src instead of the item's own slot.  own: the item's state
+// (amplitude offset), or -1 for a slot of its own in the round.
 struct ItemPass {
   int first = 0, last = -1;
   int64_t src = -1;
-  bool prefix = false;
+  int64_t own = -1;
 };
 
 struct Batch {
@@ -233,11 +233,12 @@ double item_cost(const BInst& in) {
 }
 
 // Appends one round's items to the flat plan arrays.  pv[k]: the passes of
-// item v[k] when it is wide (last < 0: all of them, 0..p).  Rounds after
-// round 0 keep their own states clear of the prefix states (which live from
-// one round to the next); round 0 and plain evaluation may overlap them (the
-// gamma chain restarts from |+> in round 1).
-RoundPlan plan_round(Batch& b, const std::vector<BItem>& v, std::vector<ItemPass> pv = {}, bool after_prefix = false) {
+// item v[k] when it is wide (last < 0: all of them, 0..p).  The round's own
+// states start at amplitude amps0: rounds after round 0 keep them clear of
+// the QAOA prefix states (which live from one round to the next), round 0 of
+// the VQE chains (which live within round 0); round 0 and plain evaluation
+// may overlap the QAOA prefixes (the gamma chain restarts from |+> in round 1).
+RoundPlan plan_round(Batch& b, const std::vector<BItem>& v, std::vector<ItemPass> pv = {}, int64_t amps0 = 0) {
   RoundPlan rp;
   pv.resize(v.size());
   std::vector<BItem> small;
@@ -265,7 +266,7 @@ RoundPlan plan_round(Batch& b, const std::vector<BItem>& v, std::vector<ItemPass
   rp.nwide = (int)wide.size();
   rp.tile_off = (int)b.all_wmap.size();
   int tiles = 0, maxpass = -1;
-  int64_t amps = after_prefix ? b.prefix_amps : 0;
+  int64_t amps = amps0;
   rp.vmap_off = (int)b.all_vmap.size();
   for (int j = 0; j < rp.nwide; ++j) {
     const BInst& in = b.insts[wide[j].first.inst];
@@ -275,8 +276,8 @@ RoundPlan plan_round(Batch& b, const std::vector<BItem>& v, std::vector<ItemPass
     b.all_wtile0.push_back(tiles);
     b.all_wfirst.push_back(ip.first);
     b.all_wlast.push_back(ip.last);
-    if (ip.prefix) {
-      b.all_wstate.push_back(b.pfx_off[wide[j].first.inst]);
+    if (ip.own >= 0) {
+      b.all_wstate.push_back(ip.own);
     } else {
       b.all_wstate.push_back(amps);
       amps += (int64_t)1 << in.n;
@@ -284,7 +285,7 @@ RoundPlan plan_round(Batch& b, const std::vector<BItem>& v, std::vector<ItemPass
     b.all_wsrc.push_back(ip.first > 0 ? ip.src : b.all_wstate.back());
     for (int t = 0; t < nt; ++t) {
       b.all_wmap.push_back(j);
-      if (in.kind == BK_VQE) b.all_vmap.push_back(tiles + t);
+      if (in.kind == BK_VQE && wide[j].first.out >= 0) b.all_vmap.push_back(tiles + t);
     }
     tiles += nt;
     maxpass = std::max(maxpass, ip.last);
@@ -339,6 +340,19 @@ void build_plan(Batch& b) {
       b.prefix_amps += (int64_t)1 << in.n;
     }
   }
+  // VQE (simultaneous gradient): the base circuit of a wide instance with
+  // L >= 2 layers runs as a chain of single-pass items that keep each pass's
+  // output (chain slot s = output of pass s), and the circuits shifted at
+  // layer l start at pass l from chain slot l - 1.
+  std::vector<int64_t> chain_off(ninst, -1);
+  int64_t chain_amps = 0;
+  for (int k = 0; k < ninst; ++k) {
+    const BInst& in = b.insts[k];
+    if (in.kind == BK_VQE && in.n > BATCH_LDS_QUBITS && in.p >= 2) {
+      chain_off[k] = chain_amps;
+      chain_amps += (int64_t)in.p << in.n;
+    }
+  }
   std::vector<std::vector<BItem>> rounds(b.nrounds);
   std::vector<std::vector<ItemPass>> rpass(b.nrounds);
   std::vector<BItem> base;
@@ -348,12 +362,41 @@ void build_plan(Batch& b) {
   for (int k = 0; k < ninst; ++k) {
     const BInst& in = b.insts[k];
     b.out0[k] = out;
-    rounds[0].push_back(BItem{k, 0, -1, out++});
+    std::vector<BItem>& r0 = rounds[0];
+    std::vector<ItemPass>& p0 = rpass[0];
+    p0.resize(r0.size());
+    const int64_t N = (int64_t)1 << in.n;
+    const bool chain = chain_off[k] >= 0;
+    auto slot = [&](int s) { return chain_off[k] + (int64_t)s * N; };
+    if (chain)
+      for (int s = 0; s < in.p; ++s) {  // passes 0..L-1 of the base circuit, each kept
+        ItemPass c;
+        c.first = c.last = s;
+        c.src = s > 0 ? slot(s - 1) : -1;
+        c.own = slot(s);
+        r0.push_back(BItem{k, 0, -1, -1});
+        p0.push_back(c);
+      }
+    ItemPass bp;
+    if (chain) {  // the base circuit's last pass, from the chain
+      bp.first = in.p;
+      bp.src = slot(in.p - 1);
+    }
+    r0.push_back(BItem{k, 0, -1, out++});
+    p0.push_back(bp);
     base.push_back(BItem{k, 0, -1, b.out0[k]});
     if (in.kind == BK_VQE)
       for (int j = 0; j < in.nparams; ++j) {
-        rounds[0].push_back(BItem{k, 1, j, out++});
-        rounds[0].push_back(BItem{k, 2, j, out++});
+        const int l = j / in.n;  // parameter (layer l, qubit j % n)
+        ItemPass sp;
+        if (chain && l >= 1) {
+          sp.first = l;
+          sp.src = slot(l - 1);
+        }
+        r0.push_back(BItem{k, 1, j, out++});
+        p0.push_back(sp);
+        r0.push_back(BItem{k, 2, j, out++});
+        p0.push_back(sp);
       }
   }
   for (int k = 0; k < ninst; ++k) {
@@ -369,8 +412,7 @@ void build_plan(Batch& b) {
       if (shared) {  // advance the prefix: pass l-1 at the current angles
         ItemPass adv;
         adv.first = adv.last = l - 1;
-        adv.src = b.pfx_off[k];
-        adv.prefix = true;
+        adv.src = adv.own = b.pfx_off[k];
         rv.push_back(BItem{k, 0, -1, -1});
         pv.push_back(adv);
       }
@@ -397,7 +439,7 @@ void build_plan(Batch& b) {
   b.all_wsrc.clear();
   b.all_ptile.clear();
   b.rounds.clear();
-  for (int r = 0; r < b.nrounds; ++r) b.rounds.push_back(plan_round(b, rounds[r], rpass[r], r > 0));
+  for (int r = 0; r < b.nrounds; ++r) b.rounds.push_back(plan_round(b, rounds[r], rpass[r], r > 0 ? b.prefix_amps : chain_amps));
   b.base = plan_round(b, base);
   b.planned = true;
   b.plan_on_device = false;

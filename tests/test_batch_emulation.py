@@ -171,3 +171,40 @@ def test_shared_prefix_rounds_match_shifted_circuits():
             assert abs(energy[it["out"]] - want) <= 1e-5 * (1 + sum(w)), (rnd, k, j, var, energy[it["out"]], want)
             seen_shared += first.get((k, var), 0) > 0
     assert seen_shared > 0  # some shifted circuits did start from a shared prefix
+
+
+def test_vqe_round0_chain_matches_shifted_circuits():
+    """Round 0 with VQE states above 13 qubits: the base circuit runs as a
+    chain of single-pass items keeping every pass's output, and the circuits
+    shifted at layer l start at pass l from it; every energy must equal the
+    oracle's at that one angle shifted by +-pi/2 (base: no shift)."""
+    b = Batch()
+    vc = []
+    for k, (n, L) in enumerate([(14, 3), (15, 2), (12, 2)]):
+        terms = workloads.random_pauli_hamiltonian(n, 8, 3, 700 + k)
+        prm = list(np.random.default_rng(710 + k).uniform(-math.pi, math.pi, n * L))
+        vc.append((n, L, terms, prm))
+        b.add_vqe(n, L, terms, prm)
+    e, w = workloads.erdos_renyi_graph(14, 0.5, 720)
+    b.add_qaoa(14, e, w, 2, [0.3, 0.9], [1.2, 0.4])
+    energy = batch_emu.evaluate_rounds(b, [0])
+    r = batch_emu.plan(b, 0)
+    firsts = {}
+    for it, f in zip(r["wide"], r["wfirst"][:r["nwide"]]):
+        if it["out"] >= 0:  # not the chain's single-pass items
+            firsts[(int(it["inst"]), int(it["variant"]), int(it["param"]))] = int(f)
+    checked = 0
+    for it in list(r["small"]) + list(r["wide"]):
+        k = int(it["inst"])
+        if it["out"] < 0 or k >= len(vc):
+            continue
+        n, L, terms, prm = vc[k]
+        p2 = list(prm)
+        if it["variant"]:
+            p2[int(it["param"])] += math.pi / 2 if it["variant"] == 1 else -math.pi / 2
+        want = orc.energy(n, workloads.ry_cz_ansatz(n, L, p2).to_array(), terms)
+        got = energy[it["out"]]
+        assert abs(got - want) <= 1e-5 * (1 + sum(abs(c) for c, _ in terms)), (k, int(it["param"]), got, want)
+        checked += 1
+    assert checked == sum(1 + 2 * n * L for n, L, _, _ in vc)
+    assert firsts[(0, 0, -1)] == 3 and firsts[(0, 1, 14 * 2)] == 2  # base from the chain; layer-2 shift at pass 2
