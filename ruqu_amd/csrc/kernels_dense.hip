@@ -3,17 +3,19 @@
 // A block is a 32x32 complex unitary U on qubits q0 < ... < q4.  Every group
 // of 32 amplitudes that differ only in those qubits is transformed y = U x.
 // With the real embedding W = [[Re U, -Im U], [Im U, Re U]] (64 x 64) the
-// groups form one real GEMM Y = X W^T (rows = groups, 64 columns = real and
+// groups form one real GEMM Y^T = W X^T (columns = groups, 64 rows = real and
 // imaginary parts), run on v_mfma_f32_32x32x2_f32: a wave takes 32 groups
-// (1024 amplitudes) at a time, 32 K-steps x 2 column halves = 64 matrix-core
+// (1024 amplitudes) at a time, 32 K-steps x 2 row halves = 64 matrix-core
 // instructions, 256 flop per amplitude, FP32 accumulation.
 //
 // Operand layout of v_mfma_f32_32x32x2_f32: lane l holds A[l % 32][l / 32]
 // and B[l / 32][l % 32]; its 16 results are column l % 32, rows
-// 8*(i/4) + 4*(l/32) + i%4.  Lane l of a wave therefore reads, for group row
+// 8*(i/4) + 4*(l/32) + i%4.  With A = W and B = X^T, lane l reads, for group
 // l % 32, the 16 amplitudes whose block index c has c % 2 == l / 32 (their
 // real parts feed K-steps 0..15, their imaginary parts steps 16..31), and
-// writes output amplitude c = l % 32 of 16 groups.
+// writes 16 output amplitudes of the same group: both the loads and the
+// stores of a wave's 32 lanes run over consecutive groups (contiguous
+// whenever the lowest qubits are outside the block).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -45,8 +47,8 @@ __device__ __forceinline__ uint64_t offset5(int c, const BlockQ& b) {
   return o;
 }
 
-// wt[(h*32 + s)*64 + l] = W[32h + l%32][2s + l/32] (the B operand of lane l
-// for K-step s and column half h), prepared on the host.
+// wt[(h*32 + s)*64 + l] = W[32h + l%32][2s + l/32] (the A operand of lane l
+// for K-step s and row half h), prepared on the host.
 __global__ __launch_bounds__(256) void k_dense5_f32(float2* __restrict__ a, uint64_t ntiles, BlockQ bq,
                                                     const float* __restrict__ wt) {
   const int lane = threadIdx.x & 63, row = lane & 31, half = lane >> 5;
@@ -58,15 +60,14 @@ __global__ __launch_bounds__(256) void k_dense5_f32(float2* __restrict__ a, uint
     b0[s] = wt[s * 64 + lane];
     b1[s] = wt[(32 + s) * 64 + lane];
   }
-  uint64_t off[16];
-#pragma unroll
-  for (int j = 0; j < 16; ++j) off[j] = offset5(2 * j + half, bq);
-  const uint64_t out_off = offset5(row, bq);
+  // block offsets are linear in the block index: a wave-uniform part plus
+  // the lane half's bit (bit 0 for the inputs, bit 2 for the outputs)
+  const uint64_t hin = half ? (1ull << bq.q[0]) : 0ull, hout = half ? (1ull << bq.q[2]) : 0ull;
   for (uint64_t tile = wave; tile < ntiles; tile += nwaves) {
     const uint64_t base = spread5(tile * 32 + row, bq);
     float2 x[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) x[j] = a[base + off[j]];
+    for (int j = 0; j < 16; ++j) x[j] = a[base + hin + offset5(2 * j, bq)];
     f32x16 acc0, acc1;
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
@@ -75,15 +76,14 @@ __global__ __launch_bounds__(256) void k_dense5_f32(float2* __restrict__ a, uint
     }
 #pragma unroll
     for (int s = 0; s < 32; ++s) {
-      const float av = s < 16 ? x[s].x : x[s - 16].y;
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, b0[s], acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, b1[s], acc1, 0, 0, 0);
+      const float xv = s < 16 ? x[s].x : x[s - 16].y;
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(b0[s], xv, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(b1[s], xv, acc1, 0, 0, 0);
     }
+    // acc0[i] / acc1[i]: real / imaginary part of output amplitude
+    // c = 8 (i / 4) + 4 half + i % 4 of this lane's group
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int grow = 8 * (i >> 2) + 4 * half + (i & 3);
-      a[spread5(tile * 32 + grow, bq) + out_off] = make_float2(acc0[i], acc1[i]);
-    }
+    for (int i = 0; i < 16; ++i) a[base + hout + offset5(8 * (i >> 2) + (i & 3), bq)] = make_float2(acc0[i], acc1[i]);
   }
 }
 
