@@ -48,37 +48,52 @@ __device__ __forceinline__ uint64_t offset5(int c, const BlockQ& b) {
 }
 
 // wt[(h*32 + s)*64 + l] = W[32h + l%32][2s + l/32] (the A operand of lane l
-// for K-step s and row half h), prepared on the host.
+// for K-step s and row half h), prepared on the host.  The block keeps it in
+// LDS (16 KB, read once per MFMA step), which leaves the registers to two
+// tiles of amplitudes: each wave loads its next tile while the matrix cores
+// work on the current one.
 __global__ __launch_bounds__(256) void k_dense5_f32(float2* __restrict__ a, uint64_t ntiles, BlockQ bq,
                                                     const float* __restrict__ wt) {
+  __shared__ float wl[64 * 64];
+  for (int i = threadIdx.x; i < 64 * 64; i += 256) wl[i] = wt[i];
+  __syncthreads();
   const int lane = threadIdx.x & 63, row = lane & 31, half = lane >> 5;
   const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint64_t nwaves = (uint64_t)gridDim.x * 4;
-  float b0[32], b1[32];
-#pragma unroll
-  for (int s = 0; s < 32; ++s) {
-    b0[s] = wt[s * 64 + lane];
-    b1[s] = wt[(32 + s) * 64 + lane];
-  }
   // block offsets are linear in the block index: a wave-uniform part plus
   // the lane half's bit (bit 0 for the inputs, bit 2 for the outputs)
   const uint64_t hin = half ? (1ull << bq.q[0]) : 0ull, hout = half ? (1ull << bq.q[2]) : 0ull;
+  float2 xn[16];
+  if (wave < ntiles) {
+    const uint64_t base = spread5(wave * 32 + row, bq) + hin;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) xn[j] = a[base + offset5(2 * j, bq)];
+  }
   for (uint64_t tile = wave; tile < ntiles; tile += nwaves) {
     const uint64_t base = spread5(tile * 32 + row, bq);
     float2 x[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) x[j] = a[base + hin + offset5(2 * j, bq)];
+    for (int j = 0; j < 16; ++j) x[j] = xn[j];
+    const uint64_t next = tile + nwaves;
+    if (next < ntiles) {
+      const uint64_t nb = spread5(next * 32 + row, bq) + hin;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) xn[j] = a[nb + offset5(2 * j, bq)];
+    }
     f32x16 acc0, acc1;
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       acc0[i] = 0.0f;
       acc1[i] = 0.0f;
     }
+    // W is re-read from LDS for every tile: a per-tile zero keeps the
+    // compiler from hoisting the 64 values back into registers
+    const float* __restrict__ wr = wl + lane + __builtin_amdgcn_readfirstlane((int)(tile >> 62));
 #pragma unroll
     for (int s = 0; s < 32; ++s) {
       const float xv = s < 16 ? x[s].x : x[s - 16].y;
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(b0[s], xv, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(b1[s], xv, acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(wr[s * 64], xv, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(wr[(32 + s) * 64], xv, acc1, 0, 0, 0);
     }
     // acc0[i] / acc1[i]: real / imaginary part of output amplitude
     // c = 8 (i / 4) + 4 half + i % 4 of this lane's group
