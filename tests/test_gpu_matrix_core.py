@@ -84,3 +84,27 @@ def test_autotune_picks_a_threshold(mf_threshold, tmp_path, monkeypatch):
     finally:
         lib().rq_tune_set(2, m)
         lib().rq_tune_set(3, c)
+
+
+@pytest.mark.parametrize("m", [10, 11])
+def test_matrix_core_phases_small_tiles(mf_threshold, m):
+    """Tiles of 10 and 11 qubits: one or two waves per workgroup, so fewer
+    (or no) wave bits can select among the composed unitaries."""
+    old_m = lib().rq_tune_get(2)
+    try:
+        assert lib().rq_tune_set(2, m) == 0
+        mf_threshold(0)
+        for kind, n in (("random", 14), ("mixed", 13)):
+            c = circuit(kind, n)
+            arr = c.to_array()
+            recs, mf, unitaries = packed_info(n, arr)
+            flush("m=%d %s n=%d: %d matrix-core phases (%d unitaries)" % (m, kind, n, mf, unitaries))
+            assert mf > 0
+            res = Simulator.run_with_config(c, SimConfig(precision="c32", fusion=True))
+            got = res.state.state_vector()
+            res.state.close()
+            want, _, _ = orc.run_circuit(n, arr)
+            assert max_abs(got, want) < 1e-5
+            assert max_abs(got, emulate(n, arr)) < 2e-6
+    finally:
+        lib().rq_tune_set(2, old_m)
