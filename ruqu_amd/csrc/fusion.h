@@ -47,7 +47,7 @@ struct FOp {
 
 struct FPhase {
   int32_t op_begin, op_end;
-  int32_t direct;                // 1: lanes sit on tile bits 0..5, access HBM directly
+  int32_t direct;                // 1: as first / last phase, access HBM in this phase's mapping
   int32_t regt[FUSE_R_BITS];     // tile bits holding register bits 0..3
   int32_t thrt[FUSE_MAX_M];      // tile bits holding thread-id bits 0..m-5
   int32_t mf_off;                // -1: ops on the VALU; else pool offset of the phase's matrix-core operands

@@ -6,9 +6,11 @@
 //   tile(k, t) = sum_b bit_b(k) << regt[b]  |  sum_j bit_j(t) << thrt[j],
 // k = 0..15 over the 4 register qubits.  Phases change the register qubits
 // by writing the tile to LDS and reading it back with the next mapping.
-// When a phase keeps its register qubits off tile bits 0..5, the 64 lanes of
-// a wave cover tile bits 0..5, i.e. runs of contiguous HBM, and the phase
-// reads (first phase) or writes (last phase) HBM directly.
+// A stage's first phase reads HBM and its last phase writes it directly in
+// their own mappings (phase.direct, set whenever a workgroup holds full
+// waves): the lanes hold the lowest non-register tile bits, so a thread's 16
+// accesses together cover every line they touch.  Otherwise the tile passes
+// through the natural mapping (thread bits = the low tile bits) and LDS.
 //
 // Arithmetic: FP contraction is allowed here (fused mode), so results differ
 // from the per-gate kernels at the rounding level.
