@@ -20,8 +20,8 @@ circuit's original gate applications per second.  The line also carries:
                 instances (G(n, 0.5), n ~ U{10..22}, U(0,1) weights,
                 p ~ U{1..8}) through the plain batched optimizer (run_qaoa's
                 rule, 16 iterations) and through the learned warm start
-                (network prediction, default start and per-depth constant,
-                each judged by one batched evaluation; instances whose
+                (network prediction and per-depth constant, each judged by
+                one batched evaluation; instances whose
                 expected cut is below 99 % of plain's get 6 plain updates
                 from their best angles, and those still below fall back to
                 the plain optimizer): instances/s of both and the fraction
@@ -257,7 +257,7 @@ def main():
         warmstart.run_plain(insts[:32], 1)  # warm-up of the batched kernels
         net = Network.load(params_path)
         base = warmstart.plain_baseline(insts, args.qaoa_iters)
-        cands = [net, warmstart.DefaultInit(), warmstart.ConstantPredictor.load()]
+        cands = [net, warmstart.ConstantPredictor.load()]
         r = warmstart.evaluate(cands, insts, args.qaoa_iters, 0, log=log, plain=base, rescue_iters=args.qaoa_rescue)
         qaoa = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
         qaoa["workload"] = ("held-out MaxCut: G(n,0.5) n~U{10..22} U(0,1) weights p~U{1..8}, complex-f32 states, "
