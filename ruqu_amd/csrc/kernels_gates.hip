@@ -102,6 +102,41 @@ __global__ __launch_bounds__(BLOCK) void k_1q_f32_v2(float4* __restrict__ a, uin
   }
 }
 
+// ---- 1q, f32, 1 <= q <= 6: partner vectors in the same wave ---------------
+// Each lane reads one float4 (amplitudes 2v, 2v + 1) of a contiguous run, so
+// every load and store of a wave covers 1 KB; the pair partner of vector v is
+// v ^ 2^(q-1), another lane of the same wave, whose value comes through a
+// lane shuffle.  A lane on the |0> side writes m00 a + m01 b, one on the |1>
+// side m10 a + m11 b: the same operations in the same order as k_1q_f32_v2.
+// Needs nvec (= 2^(n-1)) to be a multiple of 64: whole waves in or out.
+template <int U, bool NT>
+__global__ __launch_bounds__(BLOCK) void k_1q_f32_lx(float4* __restrict__ a, uint32_t q, uint64_t nvec,
+                                                     M2<float> m) {
+  const uint64_t g0 = flat_block() * U * BLOCK + threadIdx.x;
+  const int d = 1 << (q - 1);
+  const bool one = (threadIdx.x >> (q - 1)) & 1u;  // this vector holds the amplitudes with bit q = 1
+  const Cx<float> m0 = one ? m.m10 : m.m00, m1 = one ? m.m11 : m.m01;
+  float4 x[U];
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    const uint64_t v = g0 + (uint64_t)k * BLOCK;
+    if (v < nvec) x[k] = ldg<NT>(a + v);
+  }
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    const uint64_t v = g0 + (uint64_t)k * BLOCK;
+    if (v < nvec) {
+      const float4 y = make_float4(__shfl_xor(x[k].x, d), __shfl_xor(x[k].y, d), __shfl_xor(x[k].z, d),
+                                   __shfl_xor(x[k].w, d));
+      const float4 lo = one ? y : x[k], hi = one ? x[k] : y;  // |0> side, |1> side
+      const Cx<float> a0{lo.x, lo.y}, a1{lo.z, lo.w}, b0{hi.x, hi.y}, b1{hi.z, hi.w};
+      const Cx<float> o0 = cadd(cmul(m0, a0), cmul(m1, b0));
+      const Cx<float> o1 = cadd(cmul(m0, a1), cmul(m1, b1));
+      stg<NT>(a + v, make_float4(o0.re, o0.im, o1.re, o1.im));
+    }
+  }
+}
+
 // ---- 1q, f32, q == 0: one float4 holds the pair (a, b) --------------------
 template <int U, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_1q_f32_q0(float4* __restrict__ a, uint64_t npairs, M2<float> m) {
@@ -351,6 +386,10 @@ void launch_1q_t(State& s, uint32_t q, const rq_c64* m) {
     uint64_t np = s.n / 2;
     hipLaunchKernelGGL((k_1q_f32_q0<U, NT>), grid_for(np, (uint64_t)BLOCK * U), dim3(BLOCK), 0, s.stream,
                        (float4*)s.amps, np, to_m2<float>(m));
+  } else if (q <= 6 && s.n >= 128) {
+    uint64_t nv = s.n / 2;
+    hipLaunchKernelGGL((k_1q_f32_lx<U, NT>), grid_for(nv, (uint64_t)BLOCK * U), dim3(BLOCK), 0, s.stream,
+                       (float4*)s.amps, q, nv, to_m2<float>(m));
   } else {
     uint64_t ng = s.n / 4;
     hipLaunchKernelGGL((k_1q_f32_v2<U, NT>), grid_for(ng, (uint64_t)BLOCK * U), dim3(BLOCK), 0, s.stream,
