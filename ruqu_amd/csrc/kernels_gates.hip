@@ -188,6 +188,41 @@ __global__ __launch_bounds__(BLOCK) void k_1q_f64(double2* __restrict__ a, uint3
   }
 }
 
+// ---- 1q, f64, q <= 5: partner amplitudes in the same wave -----------------
+// Lane l reads amplitude v = (wave base) + l; its pair partner v ^ 2^q is
+// lane l ^ 2^q, read through lane shuffles, so every load and store of a wave
+// covers 1 KB.  Same operations in the same order as k_1q_f64.  Needs n to
+// be a multiple of 64 (whole waves in or out).
+__device__ __forceinline__ double shfl_xor_f64(double x, int d) {
+  const unsigned long long u = __double_as_longlong(x);
+  const int lo = __shfl_xor((int)(unsigned)u, d), hi = __shfl_xor((int)(unsigned)(u >> 32), d);
+  return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+template <int U, bool NT>
+__global__ __launch_bounds__(BLOCK) void k_1q_f64_lx(double2* __restrict__ a, uint32_t q, uint64_t n,
+                                                     M2<double> m) {
+  const uint64_t g0 = flat_block() * U * BLOCK + threadIdx.x;
+  const int d = 1 << q;
+  const bool one = (threadIdx.x >> q) & 1u;  // this amplitude has bit q = 1
+  const Cx<double> m0 = one ? m.m10 : m.m00, m1 = one ? m.m11 : m.m01;
+  double2 x[U];
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    const uint64_t v = g0 + (uint64_t)k * BLOCK;
+    if (v < n) x[k] = ldg<NT>(a + v);
+  }
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    const uint64_t v = g0 + (uint64_t)k * BLOCK;
+    if (v < n) {
+      const Cx<double> own{x[k].x, x[k].y}, oth{shfl_xor_f64(x[k].x, d), shfl_xor_f64(x[k].y, d)};
+      const Cx<double> av = one ? oth : own, bv = one ? own : oth;
+      const Cx<double> o = cadd(cmul(m0, av), cmul(m1, bv));
+      stg<NT>(a + v, make_double2(o.re, o.im));
+    }
+  }
+}
+
 // ---- dense 2q (state.rs:238-276): one quad per thread ---------------------
 template <class T, class V>
 __global__ __launch_bounds__(BLOCK) void k_2q_dense(V* __restrict__ a, uint32_t q1, uint32_t q2, uint64_t nquads,
@@ -378,7 +413,10 @@ M4<T> to_m4(const rq_c64* m) {
 
 template <int U, bool NT>
 void launch_1q_t(State& s, uint32_t q, const rq_c64* m) {
-  if (s.prec == RQ_C64) {
+  if (s.prec == RQ_C64 && q <= 5 && s.n >= 64) {
+    hipLaunchKernelGGL((k_1q_f64_lx<U, NT>), grid_for(s.n, (uint64_t)BLOCK * U), dim3(BLOCK), 0, s.stream,
+                       (double2*)s.amps, q, (uint64_t)s.n, to_m2<double>(m));
+  } else if (s.prec == RQ_C64) {
     uint64_t np = s.n / 2;
     hipLaunchKernelGGL((k_1q_f64<U, NT>), grid_for(np, (uint64_t)BLOCK * U), dim3(BLOCK), 0, s.stream,
                        (double2*)s.amps, q, np, to_m2<double>(m));
