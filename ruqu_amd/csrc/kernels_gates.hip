@@ -289,6 +289,146 @@ __global__ __launch_bounds__(BLOCK) void k_cz(V* __restrict__ a, uint32_t q1, ui
   a[i] = x;
 }
 
+// ---- CZ / CNOT as one streaming pass -------------------------------------
+// When both CZ qubits (a CNOT control) lie within the 128-byte line, every
+// line holds amplitudes the gate changes, so the gate costs a full pass
+// whichever way it is done; these kernels make it one: every lane reads and writes one contiguous 16-byte
+// vector (complex f32: amplitudes 2v, 2v + 1; complex f64: amplitude v).  A
+// CNOT partner vector is lane l ^ 2^(t - log2(amplitudes per vector)) of the
+// same wave (lane shuffle).  Same operations as k_cz / k_cnot (a product by
+// -1 + 0i; exact moves).  With `skip`, CZ lanes whose vector does not change
+// neither load nor store: that pays when the changed runs are >= 32 bytes
+// (a quarter of each line is written); shorter partial writes cost more than
+// rewriting the line.  Need the vector count to be a multiple of 64.
+template <class T>
+__device__ __forceinline__ Cx<T> cz_neg(Cx<T> x) {
+  return cmul(Cx<T>{T(-1), T(0)}, x);
+}
+__global__ __launch_bounds__(BLOCK) void k_cz_full_f32(float4* __restrict__ a, uint32_t q1, uint32_t q2,
+                                                       uint64_t nvec, bool skip) {
+  const uint64_t v = flat_block() * BLOCK + threadIdx.x;
+  const uint64_t m = (1ull << q1) | (1ull << q2), i0 = 2 * v;
+  if (v >= nvec || (skip && (i0 & m) != m && ((i0 + 1) & m) != m)) return;
+  float4 x = a[v];
+  if ((i0 & m) == m) {
+    const Cx<float> r = cz_neg(Cx<float>{x.x, x.y});
+    x.x = r.re;
+    x.y = r.im;
+  }
+  if (((i0 + 1) & m) == m) {
+    const Cx<float> r = cz_neg(Cx<float>{x.z, x.w});
+    x.z = r.re;
+    x.w = r.im;
+  }
+  a[v] = x;
+}
+__global__ __launch_bounds__(BLOCK) void k_cz_full_f64(double2* __restrict__ a, uint32_t q1, uint32_t q2,
+                                                       uint64_t n, bool skip) {
+  const uint64_t v = flat_block() * BLOCK + threadIdx.x;
+  const uint64_t m = (1ull << q1) | (1ull << q2);
+  const bool chg = (v & m) == m;
+  if (v >= n || (skip && !chg)) return;
+  const double2 x = a[v];
+  const Cx<double> r = cz_neg(Cx<double>{x.x, x.y});
+  a[v] = chg ? make_double2(r.re, r.im) : x;
+}
+// complex f32, t <= 6 (t = 0: the partner is the other half of the vector)
+__global__ __launch_bounds__(BLOCK) void k_cnot_full_f32(float4* __restrict__ a, uint32_t c, uint32_t t,
+                                                         uint64_t nvec) {
+  const uint64_t v = flat_block() * BLOCK + threadIdx.x;
+  if (v >= nvec) return;  // whole waves
+  const float4 x = a[v];
+  const uint64_t i0 = 2 * v;
+  float4 y;
+  if (t == 0) {
+    y = make_float4(x.z, x.w, x.x, x.y);
+  } else {
+    const int d = 1 << (t - 1);
+    y = make_float4(__shfl_xor(x.x, d), __shfl_xor(x.y, d), __shfl_xor(x.z, d), __shfl_xor(x.w, d));
+  }
+  const bool s0 = (i0 >> c) & 1, s1 = ((i0 + 1) >> c) & 1;  // control bit of each amplitude
+  a[v] = make_float4(s0 ? y.x : x.x, s0 ? y.y : x.y, s1 ? y.z : x.z, s1 ? y.w : x.w);
+}
+// complex f64, t <= 5
+__global__ __launch_bounds__(BLOCK) void k_cnot_full_f64(double2* __restrict__ a, uint32_t c, uint32_t t,
+                                                         uint64_t n) {
+  const uint64_t v = flat_block() * BLOCK + threadIdx.x;
+  if (v >= n) return;  // whole waves
+  const double2 x = a[v];
+  const int d = 1 << t;
+  const double2 y = make_double2(shfl_xor_f64(x.x, d), shfl_xor_f64(x.y, d));
+  const bool sel = (v >> c) & 1;  // per component: a select of whole vectors goes through scratch memory
+  a[v] = make_double2(sel ? y.x : x.x, sel ? y.y : x.y);
+}
+
+// ---- CZ / CNOT over the half the gate changes -----------------------------
+// When the higher CZ qubit (the CNOT control) h lies above the 128-byte line
+// (complex f32: h >= 4; complex f64: h >= 3), only the lines with bit h set
+// change.  These kernels walk just that half: half-space vector u maps to
+// vector ins0(u, hv) | 2^hv, hv = the bit of h in the vector index, so a wave
+// still reads 64 consecutive vectors in runs of >= 128 bytes.  A CNOT partner
+// differs in vector bit tv (t - log2(amplitudes per vector)), which is
+// half-space bit tv or tv - 1 (below / above hv): the lane shuffle uses that
+// bit.  With `skip`, CZ lanes whose vector the gate leaves alone neither load
+// nor store; that pays when the changed runs are >= 64 bytes (measured: at
+// 16 and 32 bytes the partial writes cost more than the bytes saved).  Same
+// operations as k_cz / k_cnot.  Need the half-space vector count to be a
+// multiple of 64.
+__global__ __launch_bounds__(BLOCK) void k_cz_half_f32(float4* __restrict__ a, uint32_t lo, uint32_t hi,
+                                                       uint64_t nhalf, bool skip) {
+  const uint64_t u = flat_block() * BLOCK + threadIdx.x;
+  if (u >= nhalf) return;
+  const uint64_t v = ins0(u, hi - 1) | (1ull << (hi - 1)), i0 = 2 * v;
+  if (skip && !((i0 >> lo) & 1) && !(((i0 + 1) >> lo) & 1)) return;
+  float4 x = a[v];
+  if ((i0 >> lo) & 1) {
+    const Cx<float> r = cz_neg(Cx<float>{x.x, x.y});
+    x.x = r.re;
+    x.y = r.im;
+  }
+  if (((i0 + 1) >> lo) & 1) {
+    const Cx<float> r = cz_neg(Cx<float>{x.z, x.w});
+    x.z = r.re;
+    x.w = r.im;
+  }
+  a[v] = x;
+}
+__global__ __launch_bounds__(BLOCK) void k_cz_half_f64(double2* __restrict__ a, uint32_t lo, uint32_t hi,
+                                                       uint64_t nhalf, bool skip) {
+  const uint64_t u = flat_block() * BLOCK + threadIdx.x;
+  if (u >= nhalf) return;
+  const uint64_t v = ins0(u, hi) | (1ull << hi);
+  const bool chg = (v >> lo) & 1;
+  if (skip && !chg) return;
+  const double2 x = a[v];
+  const Cx<double> r = cz_neg(Cx<double>{x.x, x.y});
+  a[v] = chg ? make_double2(r.re, r.im) : x;
+}
+// complex f32, c >= 4; lb = the half-space bit of the partner (t >= 1), <= 5
+__global__ __launch_bounds__(BLOCK) void k_cnot_half_f32(float4* __restrict__ a, uint32_t c, uint32_t t,
+                                                         uint32_t lb, uint64_t nhalf) {
+  const uint64_t u = flat_block() * BLOCK + threadIdx.x;
+  if (u >= nhalf) return;  // whole waves
+  const uint64_t v = ins0(u, c - 1) | (1ull << (c - 1));
+  const float4 x = a[v];
+  if (t == 0) {
+    a[v] = make_float4(x.z, x.w, x.x, x.y);
+  } else {
+    const int d = 1 << lb;
+    a[v] = make_float4(__shfl_xor(x.x, d), __shfl_xor(x.y, d), __shfl_xor(x.z, d), __shfl_xor(x.w, d));
+  }
+}
+// complex f64, c >= 3; lb <= 5 as above
+__global__ __launch_bounds__(BLOCK) void k_cnot_half_f64(double2* __restrict__ a, uint32_t c, uint32_t lb,
+                                                         uint64_t nhalf) {
+  const uint64_t u = flat_block() * BLOCK + threadIdx.x;
+  if (u >= nhalf) return;  // whole waves
+  const uint64_t v = ins0(u, c) | (1ull << c);
+  const double2 x = a[v];
+  const int d = 1 << lb;
+  a[v] = make_double2(shfl_xor_f64(x.x, d), shfl_xor_f64(x.y, d));
+}
+
 // ---- diagonal 2q (Rzz): a_i *= d[b1*2 + b2], f32 float4 / f64 double2 ----
 __global__ __launch_bounds__(BLOCK) void k_diag2_f32(float4* __restrict__ a, uint32_t q1, uint32_t q2,
                                                      uint64_t nvec, M4<float> d) {
@@ -484,6 +624,35 @@ void launch_apply_2q(State& s, uint32_t q1, uint32_t q2, const rq_c64* m) {
 
 void launch_cnot(State& s, uint32_t c, uint32_t t) {
   uint64_t nq4 = s.n / 4;
+  // a low control qubit touches every line: one streaming pass
+  if (s.prec == RQ_C32 && c <= 3 && t <= 6 && s.n >= 128) {
+    hipLaunchKernelGGL(k_cnot_full_f32, grid_for(s.n / 2, BLOCK), dim3(BLOCK), 0, s.stream, (float4*)s.amps, c, t,
+                       (uint64_t)(s.n / 2));
+    return;
+  }
+  if (s.prec == RQ_C64 && c <= 2 && t <= 5 && s.n >= 64) {
+    hipLaunchKernelGGL(k_cnot_full_f64, grid_for(s.n, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, c, t,
+                       (uint64_t)s.n);
+    return;
+  }
+  // a control above the line and a target within the wave: the control half
+  // only (complex f32: k_cnot measured as fast or faster from control 6 up)
+  if (s.prec == RQ_C32 && c >= 4 && c <= 5 && s.n >= 256) {
+    const uint32_t lb = t == 0 ? 0 : (t < c ? t - 1 : t - 2);
+    if (lb <= 5) {
+      hipLaunchKernelGGL(k_cnot_half_f32, grid_for(s.n / 4, BLOCK), dim3(BLOCK), 0, s.stream, (float4*)s.amps, c,
+                         t, lb, (uint64_t)(s.n / 4));
+      return;
+    }
+  }
+  if (s.prec == RQ_C64 && c >= 3 && s.n >= 128) {
+    const uint32_t lb = t < c ? t : t - 1;
+    if (lb <= 5) {
+      hipLaunchKernelGGL(k_cnot_half_f64, grid_for(s.n / 2, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, c,
+                         lb, (uint64_t)(s.n / 2));
+      return;
+    }
+  }
   if (s.prec == RQ_C64)
     hipLaunchKernelGGL(k_cnot<double2>, grid_for(nq4, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, c, t, nq4);
   else
@@ -500,6 +669,30 @@ void launch_swap(State& s, uint32_t q1, uint32_t q2) {
 
 void launch_cz(State& s, uint32_t q1, uint32_t q2) {
   uint64_t nq4 = s.n / 4;
+  const uint32_t lo = q1 < q2 ? q1 : q2, hi = q1 < q2 ? q2 : q1;
+  // bytes in one run of changed amplitudes (2^lo amplitudes)
+  const uint64_t run = (uint64_t)(s.prec == RQ_C32 ? 8 : 16) << lo;
+  // a low qubit and one above the line: the half with the high bit set
+  if (s.prec == RQ_C32 && lo <= 3 && hi >= 4 && s.n >= 256) {
+    hipLaunchKernelGGL(k_cz_half_f32, grid_for(s.n / 4, BLOCK), dim3(BLOCK), 0, s.stream, (float4*)s.amps, lo, hi,
+                       (uint64_t)(s.n / 4), run >= 64);
+    return;
+  }
+  if (s.prec == RQ_C64 && lo <= 2 && hi >= 3 && s.n >= 128) {
+    hipLaunchKernelGGL(k_cz_half_f64, grid_for(s.n / 2, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, lo, hi,
+                       (uint64_t)(s.n / 2), run >= 64);
+    return;
+  }
+  if (s.prec == RQ_C32 && lo <= 3 && s.n >= 128) {
+    hipLaunchKernelGGL(k_cz_full_f32, grid_for(s.n / 2, BLOCK), dim3(BLOCK), 0, s.stream, (float4*)s.amps, q1, q2,
+                       (uint64_t)(s.n / 2), run >= 32);
+    return;
+  }
+  if (s.prec == RQ_C64 && lo <= 2 && s.n >= 64) {
+    hipLaunchKernelGGL(k_cz_full_f64, grid_for(s.n, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, q1, q2,
+                       (uint64_t)s.n, run >= 32);
+    return;
+  }
   if (s.prec == RQ_C64)
     hipLaunchKernelGGL((k_cz<double, double2>), grid_for(nq4, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, q1,
                        q2, nq4);
