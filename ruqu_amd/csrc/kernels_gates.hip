@@ -429,6 +429,92 @@ __global__ __launch_bounds__(BLOCK) void k_cnot_half_f64(double2* __restrict__ a
   a[v] = make_double2(shfl_xor_f64(x.x, d), shfl_xor_f64(x.y, d));
 }
 
+// ---- SWAP with a qubit inside the 128-byte line --------------------------
+// Every line then holds amplitudes that move, so the gate is a streaming pass.
+// k_swap_wave_*: both qubits inside the wave's vectors (complex f32 <= 6,
+// complex f64 <= 5); a lane whose vector holds amplitudes with the two bits
+// unequal takes its partner vector from lane l ^ (2^L | 2^H) (L, H the two
+// bits' vector bits; the shuffle distance varies per lane).  Complex f32 with
+// lo = 0: the two amplitudes of a vector differ in bit 0, so one of them moves
+// to / from the other half of partner vector v ^ 2^(hi-1).
+// k_swap_two_*: the high qubit above the wave; lane u holds vectors x (bit
+// hi = 0) and y (bit hi = 1) of two contiguous streams.  The amplitudes that
+// move are those of x with bit lo = 1 and of y with bit lo = 0: each lane sends
+// its partner (lane ^ 2^L) the vector the partner needs and receives the one
+// it needs, so one shuffle of one vector per lane.  With `skip` a lane stores
+// only the vector that changed (worth it for changed runs >= 64 bytes).
+// Moves only: bit-exact.  Need the lane count to be a multiple of 64.
+__global__ __launch_bounds__(BLOCK) void k_swap_wave_f32(float4* __restrict__ a, uint32_t lo, uint32_t hi,
+                                                         uint64_t nvec) {
+  const uint64_t v = flat_block() * BLOCK + threadIdx.x;
+  if (v >= nvec) return;  // whole waves
+  const float4 x = a[v];
+  if (lo == 0) {
+    const int d = 1 << (hi - 1);
+    const float4 p = make_float4(__shfl_xor(x.x, d), __shfl_xor(x.y, d), __shfl_xor(x.z, d), __shfl_xor(x.w, d));
+    // bit hi = 1: amplitude 2v (bit 0 = 0) moves; bit hi = 0: amplitude 2v + 1
+    const bool h = (v >> (hi - 1)) & 1;
+    a[v] = make_float4(h ? p.z : x.x, h ? p.w : x.y, h ? x.z : p.x, h ? x.w : p.y);
+    return;
+  }
+  const uint32_t bl = (v >> (lo - 1)) & 1, bh = (v >> (hi - 1)) & 1;
+  const int d = bl != bh ? (1 << (lo - 1)) | (1 << (hi - 1)) : 0;
+  a[v] = make_float4(__shfl_xor(x.x, d), __shfl_xor(x.y, d), __shfl_xor(x.z, d), __shfl_xor(x.w, d));
+}
+__global__ __launch_bounds__(BLOCK) void k_swap_wave_f64(double2* __restrict__ a, uint32_t lo, uint32_t hi,
+                                                         uint64_t n) {
+  const uint64_t v = flat_block() * BLOCK + threadIdx.x;
+  if (v >= n) return;  // whole waves
+  const double2 x = a[v];
+  const uint32_t bl = (v >> lo) & 1, bh = (v >> hi) & 1;
+  const int d = bl != bh ? (1 << lo) | (1 << hi) : 0;
+  a[v] = make_double2(shfl_xor_f64(x.x, d), shfl_xor_f64(x.y, d));
+}
+// complex f32, lo <= 6, hi >= 7
+__global__ __launch_bounds__(BLOCK) void k_swap_two_f32(float4* __restrict__ a, uint32_t lo, uint32_t hi,
+                                                        uint64_t npairs, bool skip) {
+  const uint64_t u = flat_block() * BLOCK + threadIdx.x;
+  if (u >= npairs) return;  // whole waves
+  const uint64_t v0 = ins0(u, hi - 1), v1 = v0 | (1ull << (hi - 1));
+  const float4 x = a[v0], y = a[v1];
+  if (lo == 0) {  // x.hi (bit 0 = 1, bit hi = 0) <-> y.lo (bit 0 = 0, bit hi = 1)
+    a[v0] = make_float4(x.x, x.y, y.x, y.y);
+    a[v1] = make_float4(x.z, x.w, y.z, y.w);
+    return;
+  }
+  const int d = 1 << (lo - 1);
+  const bool one = (u >> (lo - 1)) & 1;  // bit lo of both vectors
+  // what the partner lane needs (per component: a select of whole vectors goes through scratch memory)
+  const float4 s = make_float4(one ? x.x : y.x, one ? x.y : y.y, one ? x.z : y.z, one ? x.w : y.w);
+  const float4 r = make_float4(__shfl_xor(s.x, d), __shfl_xor(s.y, d), __shfl_xor(s.z, d), __shfl_xor(s.w, d));
+  if (one) {
+    a[v0] = r;  // x's amplitudes with bit lo = 1 take y's with bit lo = 0
+    if (!skip) a[v1] = y;
+  } else {
+    if (!skip) a[v0] = x;
+    a[v1] = r;
+  }
+}
+// complex f64, lo <= 5, hi >= 6
+__global__ __launch_bounds__(BLOCK) void k_swap_two_f64(double2* __restrict__ a, uint32_t lo, uint32_t hi,
+                                                        uint64_t npairs, bool skip) {
+  const uint64_t u = flat_block() * BLOCK + threadIdx.x;
+  if (u >= npairs) return;  // whole waves
+  const uint64_t v0 = ins0(u, hi), v1 = v0 | (1ull << hi);
+  const double2 x = a[v0], y = a[v1];
+  const int d = 1 << lo;
+  const bool one = (u >> lo) & 1;
+  const double2 s = make_double2(one ? x.x : y.x, one ? x.y : y.y);
+  const double2 r = make_double2(shfl_xor_f64(s.x, d), shfl_xor_f64(s.y, d));
+  if (one) {
+    a[v0] = r;
+    if (!skip) a[v1] = y;
+  } else {
+    if (!skip) a[v0] = x;
+    a[v1] = r;
+  }
+}
+
 // ---- diagonal 2q (Rzz): a_i *= d[b1*2 + b2], f32 float4 / f64 double2 ----
 __global__ __launch_bounds__(BLOCK) void k_diag2_f32(float4* __restrict__ a, uint32_t q1, uint32_t q2,
                                                      uint64_t nvec, M4<float> d) {
@@ -661,6 +747,27 @@ void launch_cnot(State& s, uint32_t c, uint32_t t) {
 
 void launch_swap(State& s, uint32_t q1, uint32_t q2) {
   uint64_t nq4 = s.n / 4;
+  const uint32_t lo = q1 < q2 ? q1 : q2, hi = q1 < q2 ? q2 : q1;
+  const uint64_t run = (uint64_t)(s.prec == RQ_C32 ? 8 : 16) << lo;  // bytes per run of moved amplitudes
+  // a qubit inside the 128-byte line: every line changes, one streaming pass
+  if (s.prec == RQ_C32 && lo <= 3 && s.n >= 256) {
+    if (hi <= 6)
+      hipLaunchKernelGGL(k_swap_wave_f32, grid_for(s.n / 2, BLOCK), dim3(BLOCK), 0, s.stream, (float4*)s.amps, lo, hi,
+                         (uint64_t)(s.n / 2));
+    else
+      hipLaunchKernelGGL(k_swap_two_f32, grid_for(s.n / 4, BLOCK), dim3(BLOCK), 0, s.stream, (float4*)s.amps, lo, hi,
+                         (uint64_t)(s.n / 4), run >= 64);
+    return;
+  }
+  if (s.prec == RQ_C64 && lo <= 2 && s.n >= 128) {
+    if (hi <= 5)
+      hipLaunchKernelGGL(k_swap_wave_f64, grid_for(s.n, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, lo, hi,
+                         (uint64_t)s.n);
+    else
+      hipLaunchKernelGGL(k_swap_two_f64, grid_for(s.n / 2, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, lo,
+                         hi, (uint64_t)(s.n / 2), run >= 64);
+    return;
+  }
   if (s.prec == RQ_C64)
     hipLaunchKernelGGL(k_swap<double2>, grid_for(nq4, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, q1, q2, nq4);
   else

@@ -57,19 +57,19 @@ def test_every_gate_bit_exact(prec):
 
 @pytest.mark.parametrize("prec", ["c64", "c32"])
 @pytest.mark.parametrize("n", [7, 11])
-def test_low_qubit_cnot_cz_streaming_bit_exact(prec, n):
-    """CNOT and CZ with a qubit inside the 128-byte line or a target inside
+def test_low_qubit_cnot_cz_swap_streaming_bit_exact(prec, n):
+    """CNOT, CZ and SWAP with a qubit inside the 128-byte line or a target inside
     the wave run as streaming passes with lane shuffles, over the whole state
     or over the half with the high bit set: every pair with a first qubit
     below 9 against the oracle."""
-    flush("low-qubit cnot/cz %s n=%d" % (prec, n))
+    flush("low-qubit cnot/cz/swap %s n=%d" % (prec, n))
     psi = random_state(n, 7, NP[prec])
     st = QuantumState.from_amplitudes(psi, n, prec)
     for q1 in range(min(n, 9)):
         for q2 in range(n):
             if q1 == q2:
                 continue
-            for g in (Gate.CNOT(q1, q2), Gate.CZ(q1, q2)):
+            for g in (Gate.CNOT(q1, q2), Gate.CZ(q1, q2), Gate.SWAP(q1, q2)):
                 st.set_amplitudes(psi)
                 st.apply_gate(g)
                 want = psi.copy()
