@@ -332,13 +332,17 @@ __global__ __launch_bounds__(BLOCK) void k_cz_full_f64(double2* __restrict__ a, 
   const Cx<double> r = cz_neg(Cx<double>{x.x, x.y});
   a[v] = chg ? make_double2(r.re, r.im) : x;
 }
-// complex f32, t <= 6 (t = 0: the partner is the other half of the vector)
+// complex f32, t <= 6 (t = 0: the partner is the other half of the vector).
+// With `skip` (c >= 1), lanes whose vector has the control bit clear neither
+// load nor store: their partners have it clear too, so nothing reads the
+// value they pass through the shuffle.
 __global__ __launch_bounds__(BLOCK) void k_cnot_full_f32(float4* __restrict__ a, uint32_t c, uint32_t t,
-                                                         uint64_t nvec) {
+                                                         uint64_t nvec, bool skip) {
   const uint64_t v = flat_block() * BLOCK + threadIdx.x;
   if (v >= nvec) return;  // whole waves
-  const float4 x = a[v];
   const uint64_t i0 = 2 * v;
+  const bool idle = skip && !((i0 >> c) & 1);
+  const float4 x = idle ? make_float4(0.f, 0.f, 0.f, 0.f) : a[v];
   float4 y;
   if (t == 0) {
     y = make_float4(x.z, x.w, x.x, x.y);
@@ -347,18 +351,53 @@ __global__ __launch_bounds__(BLOCK) void k_cnot_full_f32(float4* __restrict__ a,
     y = make_float4(__shfl_xor(x.x, d), __shfl_xor(x.y, d), __shfl_xor(x.z, d), __shfl_xor(x.w, d));
   }
   const bool s0 = (i0 >> c) & 1, s1 = ((i0 + 1) >> c) & 1;  // control bit of each amplitude
-  a[v] = make_float4(s0 ? y.x : x.x, s0 ? y.y : x.y, s1 ? y.z : x.z, s1 ? y.w : x.w);
+  if (!idle) a[v] = make_float4(s0 ? y.x : x.x, s0 ? y.y : x.y, s1 ? y.z : x.z, s1 ? y.w : x.w);
 }
-// complex f64, t <= 5
+// complex f64, t <= 5; `skip` as above
 __global__ __launch_bounds__(BLOCK) void k_cnot_full_f64(double2* __restrict__ a, uint32_t c, uint32_t t,
-                                                         uint64_t n) {
+                                                         uint64_t n, bool skip) {
   const uint64_t v = flat_block() * BLOCK + threadIdx.x;
   if (v >= n) return;  // whole waves
-  const double2 x = a[v];
+  const bool idle = skip && !((v >> c) & 1);
+  const double2 x = idle ? make_double2(0.0, 0.0) : a[v];
   const int d = 1 << t;
   const double2 y = make_double2(shfl_xor_f64(x.x, d), shfl_xor_f64(x.y, d));
   const bool sel = (v >> c) & 1;  // per component: a select of whole vectors goes through scratch memory
-  a[v] = make_double2(sel ? y.x : x.x, sel ? y.y : x.y);
+  if (!idle) a[v] = make_double2(sel ? y.x : x.x, sel ? y.y : x.y);
+}
+
+// CNOT with the control inside the line and the target above the wave: lane u
+// holds vectors x (bit t = 0) and y (bit t = 1) of two contiguous streams and
+// exchanges the amplitudes whose control bit is set.  With `skip` (complex
+// f64) lanes whose control bit is clear do nothing.
+// Moves only: bit-exact.  Selects are per component (a select of whole vectors
+// goes through scratch memory).
+__global__ __launch_bounds__(BLOCK) void k_cnot_two_f32(float4* __restrict__ a, uint32_t c, uint32_t t,
+                                                        uint64_t npairs) {
+  const uint64_t u = flat_block() * BLOCK + threadIdx.x;
+  if (u >= npairs) return;
+  const uint64_t v0 = ins0(u, t - 1), v1 = v0 | (1ull << (t - 1));
+  if (c == 0) {  // the odd amplitude of each vector
+    const float4 x = a[v0], y = a[v1];
+    a[v0] = make_float4(x.x, x.y, y.z, y.w);
+    a[v1] = make_float4(y.x, y.y, x.z, x.w);
+    return;
+  }
+  const bool on = (v0 >> (c - 1)) & 1;
+  const float4 x = a[v0], y = a[v1];
+  a[v0] = make_float4(on ? y.x : x.x, on ? y.y : x.y, on ? y.z : x.z, on ? y.w : x.w);
+  a[v1] = make_float4(on ? x.x : y.x, on ? x.y : y.y, on ? x.z : y.z, on ? x.w : y.w);
+}
+__global__ __launch_bounds__(BLOCK) void k_cnot_two_f64(double2* __restrict__ a, uint32_t c, uint32_t t,
+                                                        uint64_t npairs, bool skip) {
+  const uint64_t u = flat_block() * BLOCK + threadIdx.x;
+  if (u >= npairs) return;
+  const uint64_t v0 = ins0(u, t), v1 = v0 | (1ull << t);
+  const bool on = (v0 >> c) & 1;
+  if (skip && !on) return;
+  const double2 x = a[v0], y = a[v1];
+  a[v0] = make_double2(on ? y.x : x.x, on ? y.y : x.y);
+  a[v1] = make_double2(on ? x.x : y.x, on ? x.y : y.y);
 }
 
 // ---- CZ / CNOT over the half the gate changes -----------------------------
@@ -711,14 +750,28 @@ void launch_apply_2q(State& s, uint32_t q1, uint32_t q2, const rq_c64* m) {
 void launch_cnot(State& s, uint32_t c, uint32_t t) {
   uint64_t nq4 = s.n / 4;
   // a low control qubit touches every line: one streaming pass
+  // bytes per run of amplitudes with the control set: from 64 up, lanes with
+  // the control clear skip their load and store
+  const bool skip = ((uint64_t)(s.prec == RQ_C32 ? 8 : 16) << c) >= 64;
   if (s.prec == RQ_C32 && c <= 3 && t <= 6 && s.n >= 128) {
     hipLaunchKernelGGL(k_cnot_full_f32, grid_for(s.n / 2, BLOCK), dim3(BLOCK), 0, s.stream, (float4*)s.amps, c, t,
-                       (uint64_t)(s.n / 2));
+                       (uint64_t)(s.n / 2), skip);
     return;
   }
   if (s.prec == RQ_C64 && c <= 2 && t <= 5 && s.n >= 64) {
     hipLaunchKernelGGL(k_cnot_full_f64, grid_for(s.n, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, c, t,
-                       (uint64_t)s.n);
+                       (uint64_t)s.n, skip);
+    return;
+  }
+  // t >= 7 (control 3: k_cnot measured faster, 2.78-3.00 against 2.83-3.05 ms)
+  if (s.prec == RQ_C32 && c <= 2 && s.n >= 256) {
+    hipLaunchKernelGGL(k_cnot_two_f32, grid_for(s.n / 4, BLOCK), dim3(BLOCK), 0, s.stream, (float4*)s.amps, c, t,
+                       (uint64_t)(s.n / 4));
+    return;
+  }
+  if (s.prec == RQ_C64 && c <= 2 && s.n >= 128) {  // t >= 6
+    hipLaunchKernelGGL(k_cnot_two_f64, grid_for(s.n / 2, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, c, t,
+                       (uint64_t)(s.n / 2), skip);
     return;
   }
   // a control above the line and a target within the wave: the control half
