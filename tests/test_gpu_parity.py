@@ -56,6 +56,26 @@ def test_every_gate_bit_exact(prec):
 
 
 @pytest.mark.parametrize("prec", ["c64", "c32"])
+@pytest.mark.parametrize("n", [12, 13])
+def test_1q_every_target_bit_exact(prec, n):
+    """Every target of a 1q gate on states large enough for each 1q kernel
+    (the pair in one vector, lane shuffles for low targets, two streams
+    above), against the oracle bit for bit."""
+    flush("1q every target %s n=%d" % (prec, n))
+    psi = random_state(n, 11, NP[prec])
+    st = QuantumState.from_amplitudes(psi, n, prec)
+    u = random_unitary(2, 9)
+    for q in range(n):
+        for g in (Gate.Ry(q, 0.731), Gate.H(q), Gate.Unitary1Q(q, u)):
+            st.set_amplitudes(psi)
+            st.apply_gate(g)
+            want = psi.copy()
+            orc.apply_gate(want, n, QuantumCircuit(n).add_gate(g).to_array()[0])
+            assert np.array_equal(st.state_vector(), want), (g, q)
+    st.close()
+
+
+@pytest.mark.parametrize("prec", ["c64", "c32"])
 @pytest.mark.parametrize("n", [7, 11])
 def test_low_qubit_cnot_cz_swap_streaming_bit_exact(prec, n):
     """CNOT, CZ and SWAP with a qubit inside the 128-byte line or a target inside
