@@ -481,14 +481,15 @@ __global__ __launch_bounds__(BLOCK) void k_cnot_half_f64(double2* __restrict__ a
 // move are those of x with bit lo = 1 and of y with bit lo = 0: each lane sends
 // its partner (lane ^ 2^L) the vector the partner needs and receives the one
 // it needs, so one shuffle of one vector per lane.  With `skip` a lane stores
-// only the vector that changed (worth it for changed runs >= 64 bytes).
+// only the vector that changed, and in k_swap_wave_* lanes whose vector does
+// not change skip load and store (worth it for changed runs >= 64 bytes).
 // Moves only: bit-exact.  Need the lane count to be a multiple of 64.
 __global__ __launch_bounds__(BLOCK) void k_swap_wave_f32(float4* __restrict__ a, uint32_t lo, uint32_t hi,
-                                                         uint64_t nvec) {
+                                                         uint64_t nvec, bool skip) {
   const uint64_t v = flat_block() * BLOCK + threadIdx.x;
   if (v >= nvec) return;  // whole waves
-  const float4 x = a[v];
   if (lo == 0) {
+    const float4 x = a[v];
     const int d = 1 << (hi - 1);
     const float4 p = make_float4(__shfl_xor(x.x, d), __shfl_xor(x.y, d), __shfl_xor(x.z, d), __shfl_xor(x.w, d));
     // bit hi = 1: amplitude 2v (bit 0 = 0) moves; bit hi = 0: amplitude 2v + 1
@@ -497,17 +498,22 @@ __global__ __launch_bounds__(BLOCK) void k_swap_wave_f32(float4* __restrict__ a,
     return;
   }
   const uint32_t bl = (v >> (lo - 1)) & 1, bh = (v >> (hi - 1)) & 1;
+  const bool idle = skip && bl == bh;  // unchanged; its partner is itself
+  const float4 x = idle ? make_float4(0.f, 0.f, 0.f, 0.f) : a[v];
   const int d = bl != bh ? (1 << (lo - 1)) | (1 << (hi - 1)) : 0;
-  a[v] = make_float4(__shfl_xor(x.x, d), __shfl_xor(x.y, d), __shfl_xor(x.z, d), __shfl_xor(x.w, d));
+  const float4 y = make_float4(__shfl_xor(x.x, d), __shfl_xor(x.y, d), __shfl_xor(x.z, d), __shfl_xor(x.w, d));
+  if (!idle) a[v] = y;
 }
 __global__ __launch_bounds__(BLOCK) void k_swap_wave_f64(double2* __restrict__ a, uint32_t lo, uint32_t hi,
-                                                         uint64_t n) {
+                                                         uint64_t n, bool skip) {
   const uint64_t v = flat_block() * BLOCK + threadIdx.x;
   if (v >= n) return;  // whole waves
-  const double2 x = a[v];
   const uint32_t bl = (v >> lo) & 1, bh = (v >> hi) & 1;
+  const bool idle = skip && bl == bh;
+  const double2 x = idle ? make_double2(0.0, 0.0) : a[v];
   const int d = bl != bh ? (1 << lo) | (1 << hi) : 0;
-  a[v] = make_double2(shfl_xor_f64(x.x, d), shfl_xor_f64(x.y, d));
+  const double2 y = make_double2(shfl_xor_f64(x.x, d), shfl_xor_f64(x.y, d));
+  if (!idle) a[v] = y;
 }
 // complex f32, lo <= 6, hi >= 7
 __global__ __launch_bounds__(BLOCK) void k_swap_two_f32(float4* __restrict__ a, uint32_t lo, uint32_t hi,
@@ -806,7 +812,7 @@ void launch_swap(State& s, uint32_t q1, uint32_t q2) {
   if (s.prec == RQ_C32 && lo <= 3 && s.n >= 256) {
     if (hi <= 6)
       hipLaunchKernelGGL(k_swap_wave_f32, grid_for(s.n / 2, BLOCK), dim3(BLOCK), 0, s.stream, (float4*)s.amps, lo, hi,
-                         (uint64_t)(s.n / 2));
+                         (uint64_t)(s.n / 2), run >= 64);
     else
       hipLaunchKernelGGL(k_swap_two_f32, grid_for(s.n / 4, BLOCK), dim3(BLOCK), 0, s.stream, (float4*)s.amps, lo, hi,
                          (uint64_t)(s.n / 4), run >= 64);
@@ -815,7 +821,7 @@ void launch_swap(State& s, uint32_t q1, uint32_t q2) {
   if (s.prec == RQ_C64 && lo <= 2 && s.n >= 128) {
     if (hi <= 5)
       hipLaunchKernelGGL(k_swap_wave_f64, grid_for(s.n, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, lo, hi,
-                         (uint64_t)s.n);
+                         (uint64_t)s.n, run >= 64);
     else
       hipLaunchKernelGGL(k_swap_two_f64, grid_for(s.n / 2, BLOCK), dim3(BLOCK), 0, s.stream, (double2*)s.amps, lo,
                          hi, (uint64_t)(s.n / 2), run >= 64);
