@@ -135,26 +135,68 @@ int plan_fused(uint32_t nq, const rq_gate* gates, size_t ngates, const FusionCon
   std::vector<int> remaining(items.size());
   for (size_t k = 0; k < items.size(); ++k) remaining[k] = (int)k;
 
-  while (!remaining.empty()) {
-    uint64_t S = low, blocked = 0;
-    std::vector<int> ops, deferred;
-    for (int id : remaining) {
+  // One greedy scan from the tile-qubit seed S0 (items in order; an item
+  // joins when its qubits are not blocked and its targets fit the tile).
+  auto scan = [&](uint64_t S0, std::vector<int>* ops, std::vector<int>* deferred, size_t window) {
+    uint64_t S = S0, blocked = 0;
+    int n_ops = 0;
+    for (size_t k = 0; k < remaining.size() && k < window; ++k) {
+      const int id = remaining[k];
       const Item& it = items[id];
-      uint64_t t = it.touched();
-      if (t & blocked) {
-        deferred.push_back(id);
-        blocked |= t;
-        continue;
-      }
-      uint64_t ns = S | it.need();
-      if (popc(ns) <= m) {
+      const uint64_t t = it.touched();
+      const uint64_t ns = S | it.need();
+      if (!(t & blocked) && popc(ns) <= m) {
         S = ns;
-        ops.push_back(id);
+        ++n_ops;
+        if (ops) ops->push_back(id);
       } else {
-        deferred.push_back(id);
         blocked |= t;
+        if (deferred) deferred->push_back(id);
       }
     }
+    return std::make_pair(n_ops, S);
+  };
+  uint64_t rng = 0x9E3779B97F4A7C15ull;  // xorshift64: fixed seed, so plans are deterministic
+  auto next_rand = [&]() {
+    rng ^= rng << 13;
+    rng ^= rng >> 7;
+    rng ^= rng << 17;
+    return rng;
+  };
+  while (!remaining.empty()) {
+    // Seed search: the stage's tile set is decided by the items that come
+    // first, so also try seeds made of the needed qubits of the front items
+    // (the first unblocked ones) in random orders, and keep the seed that
+    // takes the most items into this stage (ties: the plain scan).  Seeds are
+    // compared over the first SEED_WINDOW remaining items, which bounds the
+    // planning time.
+    uint64_t seed = low;
+    if (cfg.seeds > 0 && (int)nq > m) {
+      int front[64], nfront = 0;
+      uint64_t blocked = 0;
+      for (int id : remaining) {
+        const uint64_t nd = items[id].need() & ~low;
+        for (uint64_t b = nd & ~blocked; b && nfront < 64; b &= b - 1) {
+          const int q = __builtin_ctzll(b);
+          bool dup = false;
+          for (int k = 0; k < nfront; ++k) dup |= front[k] == q;
+          if (!dup) front[nfront++] = q;
+        }
+        blocked |= items[id].touched();
+        if (nfront >= 64 || popc(blocked) >= (int)nq) break;
+      }
+      constexpr size_t SEED_WINDOW = 384;
+      int best = scan(low, nullptr, nullptr, SEED_WINDOW).first;
+      for (int r = 0; r < cfg.seeds && nfront > m - popc(low); ++r) {
+        for (int k = nfront - 1; k > 0; --k) std::swap(front[k], front[next_rand() % (uint64_t)(k + 1)]);
+        uint64_t S0 = low;
+        for (int k = 0; k < nfront && popc(S0) < m; ++k) S0 |= 1ull << front[k];
+        const int got = scan(S0, nullptr, nullptr, SEED_WINDOW).first;
+        if (got > best) best = got, seed = S0;
+      }
+    }
+    std::vector<int> ops, deferred;
+    uint64_t S = scan(seed, &ops, &deferred, remaining.size()).second;
     // Fill the tile up to m qubits: first the qubits the deferred items need
     // (no cost now, and the next stage may start from a similar set), then the
     // lowest free qubits.
@@ -345,7 +387,7 @@ extern "C" {
 int rq_fusion_plan_info(uint32_t nq, const rq_gate* gates, size_t ngates, int m, int c, uint64_t* stages,
                         uint64_t* phases, uint64_t* ops) {
   rq::FusedPlan plan;
-  rq::FusionConfig cfg;
+  rq::FusionConfig cfg = rq::fusion_config();  // the current knobs, with this tile shape
   cfg.m = m;
   cfg.c = c;
   int rc = rq::plan_fused(nq, gates, ngates, cfg, &plan);

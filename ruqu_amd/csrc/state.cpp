@@ -833,7 +833,8 @@ extern "C" {
 // Tuning knobs (no reference counterpart).  key 0: streaming unroll (1, 2, 4,
 // 8); key 1: non-temporal loads/stores (0/1); key 2: fused tile qubits m;
 // key 3: fused low qubits c; key 4: fused matrix-core threshold (VALU
-// instructions per wave of a complex-f32 phase's ops; -1: never).
+// instructions per wave of a complex-f32 phase's ops; -1: never); key 5:
+// tile-qubit seeds the fused planner tries per stage (0: plain greedy scan).
 int rq_tune_set(int key, int value) {
   switch (key) {
     case 0:
@@ -855,6 +856,10 @@ int rq_tune_set(int key, int value) {
       if (value < -1 || value > (1 << 20)) return fail(RQ_ERR_ARGUMENT, "matrix-core threshold out of range");
       fusion_config().mf = value;
       return RQ_OK;
+    case 5:
+      if (value < 0 || value > 4096) return fail(RQ_ERR_ARGUMENT, "planner seeds out of range");
+      fusion_config().seeds = value;
+      return RQ_OK;
     default:
       return fail(RQ_ERR_ARGUMENT, "unknown tuning key");
   }
@@ -867,6 +872,7 @@ int rq_tune_get(int key) {
     case 2: return fusion_config().m;
     case 3: return fusion_config().c;
     case 4: return fusion_config().mf;
+    case 5: return fusion_config().seeds;
     default: return -2;
   }
 }
