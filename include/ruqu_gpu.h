@@ -394,7 +394,9 @@ int rq_fusion_packed_info(uint32_t num_qubits, int precision, const rq_gate* gat
  * key 1 = non-temporal HBM access (0/1), key 2 = fused tile qubits m,
  * key 3 = fused low qubits c, key 4 = fused matrix-core threshold: complex-f32
  * phases whose ops cost at least this many VALU instructions per wave run as
- * one composed 16x16 unitary on the matrix cores (-1: never).
+ * one composed 16x16 unitary on the matrix cores (-1: never), key 5 = tile-qubit
+ * seeds the fused planner tries per stage (0..4096; 0: plain greedy scan; the
+ * plan stays valid for any value, only the stage count changes).
  * rq_tune_get returns -2 for an unknown key. */
 int rq_tune_set(int key, int value);
 int rq_tune_get(int key);

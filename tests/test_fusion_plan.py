@@ -98,3 +98,22 @@ def test_matrix_core_phases_with_uniform_conditions(mf_threshold):
         want, _, _ = orc.run_circuit(13, arr)
         assert max_abs(emulate(13, arr, 12, 5, "c32"), want) < 1e-5
     assert phases > 0 and unitaries > phases
+
+
+def test_planner_seed_search_knob():
+    """Tuning key 5 (planner seeds): range checks, and every seed count gives a
+    plan that the emulator runs to the oracle's state, with no more stages than
+    the plain greedy scan."""
+    old = lib().rq_tune_get(5)
+    try:
+        assert lib().rq_tune_set(5, -1) != 0 and lib().rq_tune_set(5, 5000) != 0
+        arr = workloads.random_circuit(16, 8, 7).to_array()
+        want, _, _ = orc.run_circuit(16, arr)
+        stages = {}
+        for seeds in (0, 16, 128):
+            assert lib().rq_tune_set(5, seeds) == 0 and lib().rq_tune_get(5) == seeds
+            stages[seeds] = plan_info(16, arr, 10, 5)[0]
+            assert max_abs(emulate(16, arr, 10, 5), want) < 1e-12
+        assert stages[128] <= stages[0]
+    finally:
+        lib().rq_tune_set(5, old)
