@@ -6,6 +6,7 @@ optimizer.rs), its simd.rs kernels, and the ruqu-algorithms QAOA/VQE
 drivers.  Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of
 bench.py may import this module; it is the checker, never the product.
 """
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -40,56 +41,87 @@ def build():
 _lib = None
 
 
+def load(path):
+    """ctypes handle of an oracle build at `path`, with every signature set."""
+    handle = C.CDLL(path)
+    vp, u32, sz, d, i = C.c_void_p, C.c_uint32, C.c_size_t, C.c_double, C.c_int
+    sig = {
+        "orc_matrix_1q": (i, [vp, vp]),
+        "orc_matrix_2q": (i, [vp, vp]),
+        "orc_apply_1q_f64": (None, [vp, u32, u32, vp]),
+        "orc_apply_2q_f64": (None, [vp, u32, u32, u32, vp]),
+        "orc_apply_1q_f32": (None, [vp, u32, u32, vp]),
+        "orc_apply_2q_f32": (None, [vp, u32, u32, u32, vp]),
+        "orc_state_check": (i, [u32, i, i]),
+        "orc_apply_gate": (i, [vp, i, u32, vp, vp, sz, C.POINTER(sz), vp, C.POINTER(sz)]),
+        "orc_run_circuit": (i, [vp, i, u32, vp, sz, vp, sz, vp, C.POINTER(sz), C.POINTER(sz)]),
+        "orc_run_circuit_noisy": (i, [vp, i, u32, vp, sz, d, d, d, vp, sz, vp, C.POINTER(sz), C.POINTER(sz),
+                                      C.POINTER(sz)]),
+        "orc_measure": (i, [vp, i, u32, u32, d, C.POINTER(i), C.POINTER(d)]),
+        "orc_grover_optimal_iterations": (u32, [u32, sz]),
+        "orc_run_grover": (i, [u32, vp, sz, C.c_int64, vp, vp, C.POINTER(C.c_uint64), C.POINTER(i),
+                               C.POINTER(d), C.POINTER(u32)]),
+        "orc_measure_all": (i, [vp, i, u32, vp, vp, vp]),
+        "orc_run_shots": (i, [i, u32, vp, sz, sz, vp, sz, vp]),
+        "orc_probabilities": (None, [vp, i, u32, vp]),
+        "orc_probability_of_qubit": (d, [vp, u32, u32]),
+        "orc_expectation": (d, [vp, u32, vp, vp, sz]),
+        "orc_expectation_hamiltonian": (d, [vp, u32, vp, vp, vp, vp, sz]),
+        "orc_normalize": (None, [vp, u32]),
+        "orc_fidelity": (d, [vp, vp, u32]),
+        "orc_circuit_depth": (u32, [u32, vp, sz]),
+        "orc_mat_mul_2x2": (None, [vp, vp, vp]),
+        "orc_fuse_gates": (sz, [vp, sz, vp]),
+        "orc_build_qaoa_circuit": (sz, [u32, vp, vp, vp, sz, vp, vp, u32, vp]),
+        "orc_cut_value": (d, [vp, vp, vp, sz, vp]),
+        "orc_qaoa_cost": (i, [u32, vp, vp, vp, sz, vp, vp, u32, C.POINTER(d), vp]),
+        "orc_run_qaoa": (i, [u32, vp, vp, vp, sz, u32, u32, d, vp, vp, vp, C.POINTER(QaoaResult)]),
+        "orc_build_ansatz": (sz, [u32, u32, vp, vp]),
+        "orc_energy": (i, [u32, vp, sz, vp, vp, vp, vp, sz, C.POINTER(d)]),
+        "orc_run_vqe": (i, [u32, u32, vp, vp, vp, vp, sz, u32, d, d, vp, vp, C.POINTER(VqeResult)]),
+        "orc_simd_apply_1q_best": (None, [vp, u32, u32, vp, i]),
+        "orc_simd_apply_2q_best": (None, [vp, u32, u32, u32, vp, i]),
+        "orc_simd_apply_1q_avx2": (None, [vp, u32, u32, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(handle, name)
+        f.restype = res
+        f.argtypes = args
+    return handle
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        _lib = C.CDLL(LIB_PATH)
-        vp, u32, sz, d, i = C.c_void_p, C.c_uint32, C.c_size_t, C.c_double, C.c_int
-        sig = {
-            "orc_matrix_1q": (i, [vp, vp]),
-            "orc_matrix_2q": (i, [vp, vp]),
-            "orc_apply_1q_f64": (None, [vp, u32, u32, vp]),
-            "orc_apply_2q_f64": (None, [vp, u32, u32, u32, vp]),
-            "orc_apply_1q_f32": (None, [vp, u32, u32, vp]),
-            "orc_apply_2q_f32": (None, [vp, u32, u32, u32, vp]),
-            "orc_state_check": (i, [u32, i, i]),
-            "orc_apply_gate": (i, [vp, i, u32, vp, vp, sz, C.POINTER(sz), vp, C.POINTER(sz)]),
-            "orc_run_circuit": (i, [vp, i, u32, vp, sz, vp, sz, vp, C.POINTER(sz), C.POINTER(sz)]),
-            "orc_run_circuit_noisy": (i, [vp, i, u32, vp, sz, d, d, d, vp, sz, vp, C.POINTER(sz), C.POINTER(sz),
-                                          C.POINTER(sz)]),
-            "orc_measure": (i, [vp, i, u32, u32, d, C.POINTER(i), C.POINTER(d)]),
-            "orc_grover_optimal_iterations": (u32, [u32, sz]),
-            "orc_run_grover": (i, [u32, vp, sz, C.c_int64, vp, vp, C.POINTER(C.c_uint64), C.POINTER(i),
-                                   C.POINTER(d), C.POINTER(u32)]),
-            "orc_measure_all": (i, [vp, i, u32, vp, vp, vp]),
-            "orc_run_shots": (i, [i, u32, vp, sz, sz, vp, sz, vp]),
-            "orc_probabilities": (None, [vp, i, u32, vp]),
-            "orc_probability_of_qubit": (d, [vp, u32, u32]),
-            "orc_expectation": (d, [vp, u32, vp, vp, sz]),
-            "orc_expectation_hamiltonian": (d, [vp, u32, vp, vp, vp, vp, sz]),
-            "orc_normalize": (None, [vp, u32]),
-            "orc_fidelity": (d, [vp, vp, u32]),
-            "orc_circuit_depth": (u32, [u32, vp, sz]),
-            "orc_mat_mul_2x2": (None, [vp, vp, vp]),
-            "orc_fuse_gates": (sz, [vp, sz, vp]),
-            "orc_build_qaoa_circuit": (sz, [u32, vp, vp, vp, sz, vp, vp, u32, vp]),
-            "orc_cut_value": (d, [vp, vp, vp, sz, vp]),
-            "orc_qaoa_cost": (i, [u32, vp, vp, vp, sz, vp, vp, u32, C.POINTER(d), vp]),
-            "orc_run_qaoa": (i, [u32, vp, vp, vp, sz, u32, u32, d, vp, vp, vp, C.POINTER(QaoaResult)]),
-            "orc_build_ansatz": (sz, [u32, u32, vp, vp]),
-            "orc_energy": (i, [u32, vp, sz, vp, vp, vp, vp, sz, C.POINTER(d)]),
-            "orc_run_vqe": (i, [u32, u32, vp, vp, vp, vp, sz, u32, d, d, vp, vp, C.POINTER(VqeResult)]),
-            "orc_simd_apply_1q_best": (None, [vp, u32, u32, vp, i]),
-            "orc_simd_apply_2q_best": (None, [vp, u32, u32, u32, vp, i]),
-            "orc_simd_apply_1q_avx2": (None, [vp, u32, u32, vp]),
-        }
-        for name, (res, args) in sig.items():
-            f = getattr(_lib, name)
-            f.restype = res
-            f.argtypes = args
+        _lib = load(LIB_PATH)
     return _lib
+
+
+@contextlib.contextmanager
+def planted(path):
+    """Route every wrapper in this module through another oracle build (a
+    mutant compiled with -DORC_PLANT=k by the pinning tests) while the block
+    runs; the faithful build is restored afterwards."""
+    global _lib
+    saved = lib()
+    _lib = load(path)
+    try:
+        yield
+    finally:
+        _lib = saved
+
+
+def build_planted(k, out_dir):
+    """Compile the scalar restatement with ORC_PLANT=k (one deliberate
+    mistake, listed at the top of ruqu_oracle.cpp) plus the simd restatement
+    into out_dir/liboracle_plant<k>.so; returns the path."""
+    out = os.path.join(out_dir, "liboracle_plant%d.so" % k)
+    flags = ["-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared", "-pthread"]
+    subprocess.run(["g++"] + flags + ["-DORC_PLANT=%d" % k, os.path.join(HERE, "ruqu_oracle.cpp"),
+                                      os.path.join(HERE, "_build", "simd.o"), "-o", out], check=True)
+    return out
 
 
 def _p(a):

@@ -23,6 +23,27 @@
 #include <cstring>
 #include <vector>
 
+// Mutation testing of the pinning tests.  ORC_PLANT = 0 (the default, and the
+// only value the Makefile builds) is the faithful restatement.  A test build
+// with ORC_PLANT = k plants one deliberate mistake in the optimizer loops or
+// the ansatz builder, so the pinning tests can show that each of their checks
+// fails on a plausible error:
+//   1 run_qaoa: gamma step with the wrong sign
+//   2 run_qaoa: gamma gradients all taken at the old gammas (not in sequence)
+//   3 run_qaoa: shift applied to the next parameter instead of parameter i
+//   4 run_qaoa: argmax keeps the first maximal index instead of the last
+//   5 run_qaoa: convergence measured against the first cost, not the previous
+//   6 run_qaoa: best bitstring taken after the update (at the new angles)
+//   7 run_vqe: descent step with the wrong sign
+//   8 run_vqe: parameters updated one by one (gradient not simultaneous)
+//   9 run_vqe: best parameters tracked at the maximum energy
+//  10 build_ansatz: CNOT chain reversed (q+1 -> q)
+//  11 build_ansatz: Rz layer before the Ry layer
+//  12 build_ansatz: parameter index restarts at 0 in every layer
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+
 namespace {
 
 // ---------------------------------------------------------------------------
@@ -838,29 +859,39 @@ int orc_run_qaoa(uint32_t nq, const uint32_t* ei, const uint32_t* ej, const doub
   size_t n = (size_t)1 << nq;
   std::vector<double> amps(2 * n);
   const double shift = 1.57079632679489661923132169163975144;  // FRAC_PI_2
+  auto argmax_last = [&](const double* amps_in) {  // :246-253, Iterator::max_by keeps the LAST maximum
+    const C64* a = (const C64*)amps_in;
+    double bp = norm_sq(a[0]);
+    uint64_t k = 0;
+    for (size_t i = 1; i < n; ++i) {
+      double pi = norm_sq(a[i]);
+      if (ORC_PLANT == 4 ? (pi > bp) : !(pi < bp)) { bp = pi; k = i; }
+    }
+    return k;
+  };
   for (uint32_t it = 0; it < max_iter; ++it) {
     double cost;
     int rc = orc_qaoa_cost(nq, ei, ej, w, ne, g.data(), b.data(), p, &cost, amps.data());
     if (rc) return rc;
     history[hist++] = cost;
-    if (cost > best) {  // :238-259; argmax keeps the LAST maximal index (Iterator::max_by)
+    bool improved = cost > best;
+    if (improved) {  // :238-259
       best = cost;
-      const C64* a = (const C64*)amps.data();
-      double bp = norm_sq(a[0]);
-      best_idx = 0;
-      for (size_t i = 1; i < n; ++i) {
-        double pi = norm_sq(a[i]);
-        if (!(pi < bp)) { bp = pi; best_idx = i; }
-      }
+      best_idx = argmax_last(amps.data());
     }
-    if (it > 0 && std::fabs(cost - history[it - 1]) < 1e-6) { conv = true; break; }  // :264-270
+    const double prev = ORC_PLANT == 5 ? history[0] : (it > 0 ? history[it - 1] : 0.0);
+    if (it > 0 && std::fabs(cost - prev) < 1e-6) { conv = true; break; }  // :264-270
+    const std::vector<double> g_old = g;
     for (uint32_t i = 0; i < p; ++i) {  // :278-286
-      std::vector<double> gp = g, gm = g;
-      gp[i] += shift; gm[i] -= shift;
+      const std::vector<double>& base = ORC_PLANT == 2 ? g_old : g;
+      std::vector<double> gp = base, gm = base;
+      const uint32_t k = ORC_PLANT == 3 ? (i + 1) % p : i;
+      gp[k] += shift; gm[k] -= shift;
       double cp, cm;
       orc_qaoa_cost(nq, ei, ej, w, ne, gp.data(), b.data(), p, &cp, nullptr);
       orc_qaoa_cost(nq, ei, ej, w, ne, gm.data(), b.data(), p, &cm, nullptr);
-      g[i] += lr * (cp - cm) / 2.0;
+      if (ORC_PLANT == 1) g[i] -= lr * (cp - cm) / 2.0;
+      else g[i] += lr * (cp - cm) / 2.0;
     }
     for (uint32_t i = 0; i < p; ++i) {  // :289-297
       std::vector<double> bp = b, bm = b;
@@ -869,6 +900,11 @@ int orc_run_qaoa(uint32_t nq, const uint32_t* ei, const uint32_t* ej, const doub
       orc_qaoa_cost(nq, ei, ej, w, ne, g.data(), bp.data(), p, &cp, nullptr);
       orc_qaoa_cost(nq, ei, ej, w, ne, g.data(), bm.data(), p, &cm, nullptr);
       b[i] += lr * (cp - cm) / 2.0;
+    }
+    if (ORC_PLANT == 6 && improved) {
+      double c2;
+      orc_qaoa_cost(nq, ei, ej, w, ne, g.data(), b.data(), p, &c2, amps.data());
+      best_idx = argmax_last(amps.data());
     }
   }
   memcpy(gammas, g.data(), p * sizeof(double));
@@ -884,9 +920,16 @@ int orc_run_qaoa(uint32_t nq, const uint32_t* ei, const uint32_t* ej, const doub
 size_t orc_build_ansatz(uint32_t nq, uint32_t depth, const double* params, orc_gate* out) {
   size_t o = 0, idx = 0;
   for (uint32_t l = 0; l < depth; ++l) {
-    for (uint32_t q = 0; q < nq; ++q) { orc_gate g{}; g.kind = G_RY; g.q0 = q; g.theta = params[idx++]; out[o++] = g; }
-    for (uint32_t q = 0; q < nq; ++q) { orc_gate g{}; g.kind = G_RZ; g.q0 = q; g.theta = params[idx++]; out[o++] = g; }
-    for (uint32_t q = 0; q + 1 < nq; ++q) { orc_gate g{}; g.kind = G_CNOT; g.q0 = q; g.q1 = q + 1; out[o++] = g; }
+    if (ORC_PLANT == 12) idx = 0;
+    const int first = ORC_PLANT == 11 ? G_RZ : G_RY, second = ORC_PLANT == 11 ? G_RY : G_RZ;
+    for (uint32_t q = 0; q < nq; ++q) { orc_gate g{}; g.kind = first; g.q0 = q; g.theta = params[idx++]; out[o++] = g; }
+    for (uint32_t q = 0; q < nq; ++q) { orc_gate g{}; g.kind = second; g.q0 = q; g.theta = params[idx++]; out[o++] = g; }
+    for (uint32_t q = 0; q + 1 < nq; ++q) {
+      orc_gate g{}; g.kind = G_CNOT;
+      g.q0 = ORC_PLANT == 10 ? q + 1 : q;
+      g.q1 = ORC_PLANT == 10 ? q : q + 1;
+      out[o++] = g;
+    }
   }
   return o;
 }
@@ -932,18 +975,20 @@ int orc_run_vqe(uint32_t nq, uint32_t depth, const double* coeffs, const uint64_
     int rc = energy(params, &e);
     if (rc) return rc;
     history[hist++] = e;
-    if (e < best) { best = e; bestp = params; }
+    if (ORC_PLANT == 9 ? (hist == 1 || e > best) : e < best) { best = e; bestp = params; }
     if (it > 0 && std::fabs(history[it - 1] - e) < conv_thr) { conv = true; break; }
     std::vector<double> grad(np, 0.0);
-    for (size_t i = 0; i < np; ++i) {
+    for (size_t i = 0; i < np; ++i) {  // :188-198
       std::vector<double> pp = params, pm = params;
       pp[i] += shift; pm[i] -= shift;
       double ep, em;
       energy(pp, &ep);
       energy(pm, &em);
       grad[i] = (ep - em) / 2.0;
+      if (ORC_PLANT == 8) params[i] -= lr * grad[i];
     }
-    for (size_t i = 0; i < np; ++i) params[i] -= lr * grad[i];
+    if (ORC_PLANT != 8)  // :203-205, after every gradient entry is known
+      for (size_t i = 0; i < np; ++i) params[i] += (ORC_PLANT == 7 ? lr : -lr) * grad[i];
   }
   memcpy(best_params, bestp.data(), np * sizeof(double));
   res->optimal_energy = best;

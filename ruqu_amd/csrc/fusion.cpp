@@ -400,11 +400,11 @@ int rq_fusion_plan_info(uint32_t nq, const rq_gate* gates, size_t ngates, int m,
 
 // Runs the fused schedule of gates[0..n) on host amplitudes (rq_c64 or rq_c32
 // per precision): the program the kernel would receive, with its data mapping
-// and the current matrix-core threshold.  Host only: used to check the
+// and the current tuning knobs.  Host only: used to check the
 // planner and the packing without a GPU.
 int rq_fusion_emulate(uint32_t nq, int precision, const rq_gate* gates, size_t ngates, int m, int c, void* amps) {
   rq::FusedPlan plan;
-  rq::FusionConfig cfg;
+  rq::FusionConfig cfg = rq::fusion_config();  // the current knobs, with this tile shape
   cfg.m = m;
   cfg.c = c;
   int rc = rq::plan_fused(nq, gates, ngates, cfg, &plan);
@@ -413,12 +413,12 @@ int rq_fusion_emulate(uint32_t nq, int precision, const rq_gate* gates, size_t n
 }
 
 // Size of the program the kernel would receive for gates[0..n): op records,
-// matrix-core phases and their composed unitaries (current matrix-core
-// threshold).  Host only.
+// matrix-core phases and their composed unitaries (current tuning knobs).
+// Host only.
 int rq_fusion_packed_info(uint32_t nq, int precision, const rq_gate* gates, size_t ngates, int m, int c,
                           uint64_t* records, uint64_t* mf_phases, uint64_t* mf_unitaries) {
   rq::FusedPlan plan;
-  rq::FusionConfig cfg;
+  rq::FusionConfig cfg = rq::fusion_config();  // the current knobs, with this tile shape
   cfg.m = m;
   cfg.c = c;
   int rc = rq::plan_fused(nq, gates, ngates, cfg, &plan);
