@@ -125,6 +125,15 @@ int rq_state_write(rq_state* s, const void* host_amps);
 int rq_state_set_zero(rq_state* s);
 /* dst <- src (same size and precision). */
 int rq_state_copy(const rq_state* src, rq_state* dst);
+/* QuantumStateF32::from_f64 (mixed_precision.rs:260-278): a new complex-f32
+ * state with each amplitude of the complex-f64 state src rounded to nearest
+ * f32, src's measurement record and gate count 0.  Error: src not c64. */
+int rq_state_from_f64(const rq_state* src, rq_state** out);
+/* QuantumStateF32::to_f64 (mixed_precision.rs:280-288): a new complex-f64
+ * state with each amplitude widened exactly (through from_amplitudes, so the
+ * complex-f64 qubit limit applies) and no measurement record.  Error: src not
+ * c32. */
+int rq_state_to_f64(const rq_state* src, rq_state** out);
 /* QuantumStateF32::precision_error_bound (mixed_precision.rs:325-327);
  * 0 for c64 states. */
 double rq_precision_error_bound(const rq_state* s);

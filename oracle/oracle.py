@@ -63,6 +63,7 @@ def load(path):
                                C.POINTER(d), C.POINTER(u32)]),
         "orc_measure_all": (i, [vp, i, u32, vp, vp, vp]),
         "orc_run_shots": (i, [i, u32, vp, sz, sz, vp, sz, vp]),
+        "orc_measure_all_shots": (i, [vp, i, u32, vp, sz, vp]),
         "orc_probabilities": (None, [vp, i, u32, vp]),
         "orc_probability_of_qubit": (d, [vp, u32, u32]),
         "orc_expectation": (d, [vp, u32, vp, vp, sz]),
@@ -224,6 +225,18 @@ def measure_all(amps, n, draws):
     d = np.ascontiguousarray(draws, dtype=np.float64)
     lib().orc_measure_all(_p(amps), 1 if f32 else 0, n, _p(d), _p(res), _p(probs))
     return res, probs
+
+
+def measure_all_shots(amps, n, draws):
+    """measure_all (state.rs:329-335) on a fresh copy of `amps` for every row
+    of draws (shots x n); returns the bits (shots x n, uint8).  Memoised over
+    shared outcome prefixes, so 10^5 shots of a 26-qubit state take seconds."""
+    a = np.ascontiguousarray(amps)
+    d = np.ascontiguousarray(draws, dtype=np.float64)
+    shots = d.shape[0]
+    bits = np.zeros((shots, n), dtype=np.uint8)
+    lib().orc_measure_all_shots(_p(a), 1 if a.dtype == np.complex64 else 0, n, _p(d), shots, _p(bits))
+    return bits
 
 
 def run_shots(n, gates, shots, draws, f32=False):

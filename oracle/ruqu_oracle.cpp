@@ -616,6 +616,58 @@ int orc_measure_all(void* amps, int f32, uint32_t nq, const double* draws, int* 
   return E_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// measure_all (state.rs:329-335) of many shots of one state, memoised over the
+// outcome prefixes the shots share.  After qubits 0..k-1 collapsed to the bits
+// r, the reference's state is zero except at indices i = r + j 2^k, so `sub`
+// keeps those amplitudes (j in order) and skips the zeros: adding norm_sq(0) =
+// +0 to the non-negative running p0 leaves it unchanged, and a zero scaled by
+// the norm factor stays zero.  Every other operation is measure64 /
+// measure32's, in the same order, so each shot's bits equal those of
+// measure_all on its own copy of the state.
+template <class Amp, class Real>
+void shots_level(std::vector<Amp>& sub, uint32_t k, uint32_t nq, const double* draws, const std::vector<uint32_t>& ids,
+                 uint8_t* bits) {
+  if (k == nq || ids.empty()) return;
+  Real p0 = 0;
+  for (size_t j = 0; j < sub.size(); j += 2) p0 += sub[j].re * sub[j].re + sub[j].im * sub[j].im;
+  for (int b = 0; b < 2; ++b) {
+    std::vector<uint32_t> grp;
+    for (uint32_t s : ids)
+      if ((draws[(size_t)s * nq + k] >= (double)p0) == (b == 1)) grp.push_back(s);
+    if (grp.empty()) continue;
+    for (uint32_t s : grp) bits[(size_t)s * nq + k] = (uint8_t)b;
+    Real prob = b ? (Real)1 - p0 : p0;
+    Real nf = prob > 0 ? (Real)1 / std::sqrt(prob) : (Real)0;
+    std::vector<Amp> child(sub.size() / 2);
+    for (size_t j = 0; j < child.size(); ++j) child[j] = Amp{sub[2 * j + b].re * nf, sub[2 * j + b].im * nf};
+    shots_level<Amp, Real>(child, k + 1, nq, draws, grp, bits);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// measure_all on a copy of `amps` for every shot (draws[shot*nq + q]), bits
+// out as bits[shot*nq + q].  Same results as orc_measure_all per shot.
+int orc_measure_all_shots(const void* amps, int f32, uint32_t nq, const double* draws, size_t shots, uint8_t* bits) {
+  size_t n = (size_t)1 << nq;
+  std::vector<uint32_t> ids(shots);
+  for (size_t s = 0; s < shots; ++s) ids[s] = (uint32_t)s;
+  if (f32) {
+    std::vector<C32> sub((const C32*)amps, (const C32*)amps + n);
+    shots_level<C32, float>(sub, 0, nq, draws, ids, bits);
+  } else {
+    std::vector<C64> sub((const C64*)amps, (const C64*)amps + n);
+    shots_level<C64, double>(sub, 0, nq, draws, ids, bits);
+  }
+  return E_OK;
+}
+
 // simulator.rs:107-164 — run_shots, with draws supplied per shot
 // (draws[shot*stride ...]).  Each shot re-simulates the circuit (:123-131);
 // if the circuit has no Measure gate, measure_all follows (:133-137).

@@ -69,6 +69,8 @@ SIGNATURES = {
     "rq_state_write": (_i, [_vp, _vp]),
     "rq_state_set_zero": (_i, [_vp]),
     "rq_state_copy": (_i, [_vp, _vp]),
+    "rq_state_from_f64": (_i, [_vp, _vp]),
+    "rq_state_to_f64": (_i, [_vp, _vp]),
     "rq_precision_error_bound": (_d, [_vp]),
     "rq_apply_gate": (_i, [_vp, _vp, _d, C.POINTER(Measurement), C.POINTER(_i)]),
     "rq_apply_1q": (_i, [_vp, _u32, _vp]),

@@ -82,11 +82,16 @@ void launch_scale(State& s, double f);
 void launch_negate_from(State& s, uint64_t start);                       // a[i] = -a[i], i >= start
 void launch_negate_listed(State& s, const uint64_t* d_idx, uint64_t count);  // distinct device indices
 void launch_copy(State& dst, const State& src);
+// dst and src of the same size and opposite precisions: narrow (to c32) or widen (to c64).
+void launch_convert(State& dst, const State& src);
 void launch_plain_copy(void* dst, const void* src, size_t bytes, hipStream_t st);
 
 // Reductions: results land in s.red (device); the caller reads them back.
 // Sum over amplitudes with (i & mask) == want of |a_i|^2 (FP64 accumulate).
 int reduce_masked_normsq(State& s, uint64_t mask, uint64_t want, double* out);
+// P(q = 0) of a complex-f32 state summed sequentially in f32 in index order,
+// bit for bit as mixed_precision.rs:480-486 (one workgroup).
+int reduce_p0_seq_f32(State& s, uint32_t q, float* out);
 // Per-term <P_k> for canonicalised Pauli strings (x, zy parity mask, nY mod 4).
 struct PauliCanon {
   uint64_t x;

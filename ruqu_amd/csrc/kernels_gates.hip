@@ -713,6 +713,26 @@ void launch_copy_t(void* dst, const void* src, size_t bytes, hipStream_t st) {
                      (const float4*)src, nv);
 }
 
+// QuantumStateF32::from_f64 / to_f64 (mixed_precision.rs:260-288): each
+// amplitude narrowed with round-to-nearest (Complex32::from_f64, `as f32`) or
+// widened exactly (Complex32::to_f64).
+__global__ __launch_bounds__(BLOCK) void k_narrow(float2* __restrict__ dst, const double2* __restrict__ src,
+                                                  uint64_t n) {
+  const uint64_t i = flat_block() * BLOCK + threadIdx.x;
+  if (i < n) {
+    const double2 v = src[i];
+    dst[i] = make_float2((float)v.x, (float)v.y);
+  }
+}
+__global__ __launch_bounds__(BLOCK) void k_widen(double2* __restrict__ dst, const float2* __restrict__ src,
+                                                 uint64_t n) {
+  const uint64_t i = flat_block() * BLOCK + threadIdx.x;
+  if (i < n) {
+    const float2 v = src[i];
+    dst[i] = make_double2((double)v.x, (double)v.y);
+  }
+}
+
 }  // namespace
 
 // Shape of the streaming kernels: amplitude groups per thread (1, 2, 4, 8)
@@ -932,5 +952,14 @@ void launch_plain_copy(void* dst, const void* src, size_t bytes, hipStream_t st)
 }
 
 void launch_copy(State& dst, const State& src) { launch_plain_copy(dst.amps, src.amps, src.bytes(), dst.stream); }
+
+void launch_convert(State& dst, const State& src) {
+  if (dst.prec == RQ_C32)
+    hipLaunchKernelGGL(k_narrow, grid_for(src.n, BLOCK), dim3(BLOCK), 0, dst.stream, (float2*)dst.amps,
+                       (const double2*)src.amps, (uint64_t)src.n);
+  else
+    hipLaunchKernelGGL(k_widen, grid_for(src.n, BLOCK), dim3(BLOCK), 0, dst.stream, (double2*)dst.amps,
+                       (const float2*)src.amps, (uint64_t)src.n);
+}
 
 }  // namespace rq

@@ -4,6 +4,8 @@
 // per-thread partials, a wave-shuffle + LDS block reduction into one partial
 // per block, then one block folds the partials in a fixed order, so results
 // are deterministic run to run.
+#include <cstring>
+
 #include "internal.h"
 
 #pragma clang fp contract(off)
@@ -146,6 +148,117 @@ __global__ __launch_bounds__(BLOCK) void k_inner(const V* __restrict__ a, const 
   if (threadIdx.x == 0) part[gridDim.x + blockIdx.x] = r2;
 }
 
+// P(qubit q = 0) of a complex-f32 state as QuantumStateF32::measure computes
+// it (mixed_precision.rs:480-486): p0 = 0f32, then p0 += norm_sq(a_i) in f32,
+// in index order over the indices with bit q clear.  That chain of 2^(n-1)
+// rounded additions is reproduced bit for bit by one workgroup walking the
+// terms in chunks of P0_CHUNK:
+//  * while the running sum s is a normal float in the binade [2^E, 2^(E+1)),
+//    every float there is a multiple of u = 2^(E-23), so rounding s + x to
+//    nearest gives s + r(x) u with r(x) = x/u rounded to the nearest integer,
+//    whatever s is, unless x/u lies exactly halfway (then round-to-even looks
+//    at s).  If no term of the chunk is such a tie and s/u + sum r < 2^24
+//    (every partial sum stays inside the binade; all terms are >= 0), the
+//    chunk's chain equals s + (sum r) u: an integer sum done in parallel;
+//  * otherwise (s zero or subnormal, a tie, or the sum leaves the binade)
+//    thread 0 adds the chunk's terms one by one from LDS, as the reference.
+// Terms are norm_sq in f32 (re*re + im*im without contraction).
+constexpr int P0_THREADS = 1024;
+constexpr int P0_PER = 16;
+constexpr int P0_CHUNK = P0_THREADS * P0_PER;
+
+__device__ __forceinline__ uint64_t ins0(uint64_t k, uint32_t q) {
+  return ((k >> q) << (q + 1)) | (k & ((1ull << q) - 1));
+}
+
+__global__ __launch_bounds__(P0_THREADS) void k_p0_seq_f32(const float2* __restrict__ a, uint64_t nhalf, uint32_t q,
+                                                         float* __restrict__ out) {
+  __shared__ __align__(16) float xs[P0_CHUNK];
+  __shared__ unsigned long long wsum[P0_THREADS / WAVE];
+  __shared__ int wflag[P0_THREADS / WAVE];
+  __shared__ float s_sh;
+  const int t = threadIdx.x, lane = t & (WAVE - 1), w = t / WAVE;
+  if (t == 0) s_sh = 0.0f;
+  float2 cur[P0_PER];
+  auto load = [&](uint64_t base, float2* v) {
+#pragma unroll
+    for (int j = 0; j < P0_PER; ++j) {
+      uint64_t k = base + (uint64_t)t * P0_PER + j;
+      v[j] = k < nhalf ? a[ins0(k, q)] : make_float2(0.0f, 0.0f);
+    }
+  };
+  load(0, cur);
+  __syncthreads();
+  for (uint64_t base = 0; base < nhalf; base += P0_CHUNK) {
+    float x[P0_PER];
+#pragma unroll
+    for (int j = 0; j < P0_PER; ++j) {
+      float re2 = cur[j].x * cur[j].x, im2 = cur[j].y * cur[j].y;
+      x[j] = re2 + im2;
+    }
+    if (base + P0_CHUNK < nhalf) load(base + P0_CHUNK, cur);  // next chunk in flight
+#pragma unroll
+    for (int j = 0; j < P0_PER; ++j) xs[t * P0_PER + j] = x[j];
+    const float s = s_sh;
+    const bool normal = s >= 1.17549435e-38f;  // FLT_MIN
+    int e2 = 0;
+    (void)frexpf(s, &e2);  // s = m 2^e2, m in [0.5, 1): binade E = e2 - 1
+    const int E = e2 - 1;
+    unsigned long long rs = 0;
+    int bad = 0;
+    if (normal) {
+#pragma unroll
+      for (int j = 0; j < P0_PER; ++j) {
+        double y = ldexp((double)x[j], 23 - E);  // x / u, exact
+        if (!(y < 33554432.0)) {                 // >= 2^25 (or NaN): cannot stay in the binade
+          bad = 1;
+          continue;
+        }
+        double f = floor(y), d = y - f;
+        if (d == 0.5) bad = 1;
+        rs += (unsigned long long)f + (d > 0.5 ? 1ull : 0ull);
+      }
+    }
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) rs += __shfl_down(rs, off, WAVE);
+    bad = __any(bad);
+    if (lane == 0) {
+      wsum[w] = rs;
+      wflag[w] = bad;
+    }
+    __syncthreads();
+    if (t == 0) {
+      unsigned long long tot = 0;
+      int anybad = !normal;
+      for (int k = 0; k < P0_THREADS / WAVE; ++k) {
+        tot += wsum[k];
+        anybad |= wflag[k];
+      }
+      const unsigned long long s_units = (unsigned long long)ldexpf(s, 23 - E);  // in [2^23, 2^24)
+      float ns;
+      if (!anybad && s_units + tot < (1ull << 24)) {
+        ns = ldexpf((float)(s_units + tot), E - 23);
+      } else {
+        // terms past the end are +0 and leave the sum unchanged, so whole float4s are read
+        ns = s;
+        const uint64_t left = nhalf - base;
+        const int cnt4 = (int)((left < (uint64_t)P0_CHUNK ? left : (uint64_t)P0_CHUNK) + 3) / 4;
+        const float4* x4 = (const float4*)xs;
+        for (int j = 0; j < cnt4; ++j) {
+          float4 v = x4[j];
+          ns = ns + v.x;
+          ns = ns + v.y;
+          ns = ns + v.z;
+          ns = ns + v.w;
+        }
+      }
+      s_sh = ns;
+    }
+    __syncthreads();
+  }
+  if (t == 0) *out = s_sh;
+}
+
 // probabilities (state.rs:123-125): f64 re*re+im*im; f32 computed in f32 then
 // widened (mixed_precision.rs:303-305).
 __global__ __launch_bounds__(BLOCK) void k_probs_f64(const double2* __restrict__ a, uint64_t n, double* __restrict__ o) {
@@ -234,6 +347,18 @@ int reduce_masked_normsq(State& s, uint64_t mask, uint64_t want, double* out) {
   hipLaunchKernelGGL(k_fold, dim3(1), dim3(BLOCK), 0, s.stream, s.red + 1, (int)nb, s.red);
   RQ_HIP(hipGetLastError());
   return read_red(s, 0, 1, out);
+}
+
+int reduce_p0_seq_f32(State& s, uint32_t q, float* out) {
+  int rc = ensure_red(s, 1);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_p0_seq_f32, dim3(1), dim3(P0_THREADS), 0, s.stream, (const float2*)s.amps, (uint64_t)(s.n / 2),
+                     q, (float*)s.red);
+  RQ_HIP(hipGetLastError());
+  RQ_HIP(hipMemcpyAsync(s.host_red, s.red, sizeof(float), hipMemcpyDeviceToHost, s.stream));
+  RQ_HIP(hipStreamSynchronize(s.stream));
+  memcpy(out, s.host_red, sizeof(float));
+  return RQ_OK;
 }
 
 int reduce_pauli(State& s, const PauliCanon* terms, size_t nterms, double* out) {

@@ -90,23 +90,25 @@ int alloc_state(uint32_t nq, int prec, State** out) {
   return RQ_OK;
 }
 
-// Measurement with a supplied draw: state.rs:288-326 (f64) and
-// mixed_precision.rs:473-515 (f32: p0 rounded to f32, compared as f64,
-// 1/sqrt in f32).  p0 itself is accumulated in FP64 on the device.
+// Measurement with a supplied draw: state.rs:288-326 (f64: p0 summed in FP64)
+// and mixed_precision.rs:473-515 (f32: p0 summed sequentially in f32 in index
+// order, bit for bit, compared with the draw as f64, 1/sqrt in f32).
 int measure_impl(State& s, uint32_t q, double draw, rq_measurement* out) {
   int rc = check_qubit(s, q);
-  if (rc) return rc;
-  double p0d = 0.0;
-  rc = reduce_masked_normsq(s, 1ull << q, 0, &p0d);
   if (rc) return rc;
   int result;
   double prob, nf;
   if (s.prec == RQ_C64) {
+    double p0d = 0.0;
+    rc = reduce_masked_normsq(s, 1ull << q, 0, &p0d);
+    if (rc) return rc;
     result = draw >= p0d;
     prob = result ? 1.0 - p0d : p0d;
     nf = prob > 0.0 ? 1.0 / std::sqrt(prob) : 0.0;
   } else {
-    float p0 = (float)p0d;
+    float p0 = 0.0f;
+    rc = reduce_p0_seq_f32(s, q, &p0);
+    if (rc) return rc;
     result = draw >= (double)p0;
     float pf = result ? 1.0f - p0 : p0;
     float nff = pf > 0.0f ? 1.0f / std::sqrt(pf) : 0.0f;
@@ -140,6 +142,32 @@ int reset_impl(State& s, uint32_t q, double draw) {  // state.rs:344-352
 }  // namespace
 
 int measure_state(State& s, uint32_t q, double draw, rq_measurement* out) { return measure_impl(s, q, draw, out); }
+
+// mixed_precision.rs:260-278 (from_f64: narrowed amplitudes, the source's
+// measurement record, gate count 0) and :280-283 (to_f64: widened amplitudes
+// through QuantumState::from_amplitudes, so the qubit limit of the complex-f64
+// state applies, and no measurement record).
+int convert_state(const State& src, int to_prec, State** out) {
+  uint32_t maxq = rq_max_qubits(to_prec);
+  if (src.nq > maxq) return limit_error(src.nq, (int)maxq);
+  State* d = nullptr;
+  int rc = alloc_state(src.nq, to_prec, &d);
+  if (rc) return rc;
+  hipError_t e = hipStreamSynchronize(src.stream);
+  if (e == hipSuccess) {
+    launch_convert(*d, src);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+  if (e != hipSuccess) {
+    rq_state_destroy((rq_state*)d);
+    return hip_fail(e, "precision conversion");
+  }
+  if (to_prec == RQ_C32) d->record = src.record;
+  *out = d;
+  return RQ_OK;
+}
+
 int reset_state(State& s, uint32_t q, double draw) { return reset_impl(s, q, draw); }
 
 int fail(int code, const std::string& msg) {
@@ -476,6 +504,28 @@ int rq_state_copy(const rq_state* hs, rq_state* hd) {
   launch_copy(*dst, *src);
   RQ_HIP(hipGetLastError());
   RQ_HIP(hipStreamSynchronize(dst->stream));
+  return RQ_OK;
+}
+
+int rq_state_from_f64(const rq_state* h, rq_state** out) {
+  const State* s = (const State*)h;
+  if (!s || !out) return fail(RQ_ERR_ARGUMENT, "null pointer");
+  if (s->prec != RQ_C64) return fail(RQ_ERR_ARGUMENT, "from_f64 needs a complex-f64 state");
+  State* d = nullptr;
+  int rc = convert_state(*s, RQ_C32, &d);
+  if (rc) return rc;
+  *out = (rq_state*)d;
+  return RQ_OK;
+}
+
+int rq_state_to_f64(const rq_state* h, rq_state** out) {
+  const State* s = (const State*)h;
+  if (!s || !out) return fail(RQ_ERR_ARGUMENT, "null pointer");
+  if (s->prec != RQ_C32) return fail(RQ_ERR_ARGUMENT, "to_f64 needs a complex-f32 state");
+  State* d = nullptr;
+  int rc = convert_state(*s, RQ_C64, &d);
+  if (rc) return rc;
+  *out = (rq_state*)d;
   return RQ_OK;
 }
 

@@ -217,6 +217,23 @@ class QuantumState:
     def precision_error_bound(self):
         return float(lib().rq_precision_error_bound(self._h))
 
+    @classmethod
+    def from_f64(cls, state):
+        """QuantumStateF32::from_f64 (mixed_precision.rs:260-278): a new
+        complex-f32 state, each amplitude of the complex-f64 `state` rounded to
+        nearest, with its measurement record."""
+        h = C.c_void_p()
+        check(lib().rq_state_from_f64(state.handle, C.byref(h)))
+        return cls(h.value, C32)
+
+    def to_f64(self):
+        """QuantumStateF32::to_f64 (mixed_precision.rs:280-288): a new
+        complex-f64 state, each amplitude widened exactly, no measurement
+        record."""
+        h = C.c_void_p()
+        check(lib().rq_state_to_f64(self._h, C.byref(h)))
+        return QuantumState(h.value, C64)
+
     def copy(self):
         h = C.c_void_p()
         check(lib().rq_state_create(self.num_qubits(), self._prec, 1, C.byref(h)))

@@ -227,13 +227,11 @@ def test_config1_20q_c64():
     draws = workloads.uniforms(workloads.shot_seed(1), shots * n).reshape(shots, n)
     bits, ties = exact.state.sample_shots(draws)
     assert bits.shape == (shots, n)
-    flush("  near ties %d" % ties)
-    checked = 64
-    for s in range(checked):
-        a = want.copy()
-        res, _ = orc.measure_all(a, n, draws[s])
-        if ties == 0:
-            assert list(bits[s]) == list(res), s
+    # every shot against measure_all on a copy of the oracle's state
+    ref = orc.measure_all_shots(want, n, draws)
+    mismatch = int(np.count_nonzero(np.any(bits != ref, axis=1)))
+    flush("  %d shots: %d differ, near ties %d" % (shots, mismatch, ties))
+    assert mismatch <= ties
 
 
 @pytest.mark.parametrize("prec", ["c64", "c32"])
@@ -246,17 +244,90 @@ def test_sample_shots_match_measure_all(prec):
     shots = 2000
     draws = workloads.uniforms(77, shots * n).reshape(shots, n)
     bits, ties = res.state.sample_shots(draws)
+    # The sampler is state.rs:329-335 measure_all (FP64 p0) for both
+    # precisions; a complex-f32 state is sampled as its exact widening
+    # (QuantumStateF32::to_f64, mixed_precision.rs:280-283).
+    wide = amps.astype(np.complex128)
     mismatch = 0
     for s in range(shots):
-        a = amps.copy()
+        a = wide.copy()
         r, _ = orc.measure_all(a, n, draws[s])
         mismatch += int(list(bits[s]) != list(r))
     flush("  mismatches %d near ties %d" % (mismatch, ties))
-    if prec == "c64":
-        assert mismatch <= ties
-    else:
-        # f32: the reference accumulates p0 in f32 sequentially; ours is FP64.
-        assert mismatch <= max(ties, 5)
+    assert mismatch <= ties
+
+
+@pytest.mark.parametrize("n", [1, 2, 15, 22])
+def test_f32_measure_sequential_sum(n):
+    """QuantumStateF32::measure sums p0 in f32 in index order
+    (mixed_precision.rs:480-486); the device reproduces that sum bit for bit,
+    so outcome, probability and collapsed state equal the oracle's for draws
+    on both sides of p0 and right at it."""
+    flush("f32 measure n=%d" % n)
+    psi = random_state(n, 40 + n, np.complex64)
+    st = QuantumState.from_amplitudes(psi, n, "c32")
+    for q in sorted({0, n // 2, n - 1}):
+        a = psi.copy()
+        _, p0 = orc.measure(a, n, q, 0.0)  # draw 0: outcome 0 with probability p0
+        for draw in (0.0, float(np.nextafter(np.float32(p0), np.float32(0))), float(np.float32(p0)), 0.31, 0.97):
+            st.set_amplitudes(psi)
+            out = st.measure(q, draw)
+            a = psi.copy()
+            r, p = orc.measure(a, n, q, draw)
+            assert (out.result, out.probability) == (r, p), (q, draw)
+            assert np.array_equal(st.state_vector(), a), (q, draw)
+    st.close()
+
+
+def test_f32_measure_reproduces_the_f32_stall():
+    """26 qubits of equal magnitude and random phase: P(q=0) is 0.5, but
+    every term is about 2^-26, so the reference's f32 running sum stops near
+    0.25, where a term falls below half an ulp.  With draw 0.3 the reference
+    therefore measures 1 with probability ~0.75; the device must do the same,
+    bit for bit."""
+    n = 26
+    flush("f32 stall n=%d" % n)
+    rng = np.random.default_rng(26)
+    ph = rng.uniform(0, 2 * np.pi, 1 << n)
+    psi = (np.exp(1j * ph) * 2.0 ** (-n / 2)).astype(np.complex64)
+    st = QuantumState.from_amplitudes(psi, n, "c32")
+    a = psi.copy()
+    r, p = orc.measure(a, n, 3, 0.3)
+    out = st.measure(3, 0.3)
+    flush("  oracle result %d prob %.9f, device %d %.9f" % (r, p, out.result, out.probability))
+    assert r == 1 and p > 0.7  # the reference's f32 sum fell short of 0.5
+    assert (out.result, out.probability) == (r, p)
+    assert np.array_equal(st.state_vector(), a)
+    st.close()
+
+
+def test_precision_conversions_bit_exact():
+    """QuantumStateF32::from_f64 / to_f64 (mixed_precision.rs:260-288): each
+    amplitude rounded to nearest f32 / widened exactly; from_f64 keeps the
+    measurement record, to_f64 starts a fresh one."""
+    from ruqu_amd import ArgumentError
+    n = 14
+    flush("from_f64 / to_f64 n=%d" % n)
+    psi = random_state(n, 8)
+    s64 = QuantumState.from_amplitudes(psi, n, "c64")
+    s64.measure(3, 0.4)
+    a = psi.copy()
+    orc.measure(a, n, 3, 0.4)
+    s32 = QuantumState.from_f64(s64)
+    assert s32.precision == "c32"
+    assert np.array_equal(s32.state_vector(), a.astype(np.complex64))
+    assert [(m.qubit, m.result, m.probability) for m in s32.measurement_record()] == \
+        [(m.qubit, m.result, m.probability) for m in s64.measurement_record()]
+    back = s32.to_f64()
+    assert back.precision == "c64"
+    assert np.array_equal(back.state_vector(), a.astype(np.complex64).astype(np.complex128))
+    assert back.measurement_record() == []
+    with pytest.raises(ArgumentError):
+        QuantumState.from_f64(s32)
+    with pytest.raises(ArgumentError):
+        s64.to_f64()
+    for st in (s64, s32, back):
+        st.close()
 
 
 def test_run_shots_histogram_semantics():
@@ -284,6 +355,9 @@ def test_measure_and_reset(prec):
             a = psi.copy()
             r, p = orc.measure(a, n, q, draw)
             assert out.result == r
+            if prec == "c32":  # p0 summed in f32 in index order, as the reference: bit-exact
+                assert out.probability == p
+                assert np.array_equal(st.state_vector(), a)
             assert abs(out.probability - p) < TOL[prec]
             assert max_abs(st.state_vector(), a) < TOL[prec]
             st2 = QuantumState.from_amplitudes(psi, n, prec)

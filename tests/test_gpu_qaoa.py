@@ -108,6 +108,11 @@ def test_qaoa_26_nodes_p4_against_fixture():
           (fused, cuts.mean(), se, cuts.max(), ties), flush=True)
     assert abs(cuts.mean() - fused) < 6 * se
     assert np.array_equal(cut_values(g, bits[:3]), [sum(1 for i, j in edges if b[i] != b[j]) for b in bits[:3]])
+    # every one of the 10^5 shots against measure_all on a copy of the exact state
+    ref = orc.measure_all_shots(amps, n, draws)
+    mismatch = int(np.count_nonzero(np.any(bits != ref, axis=1)))
+    print("  %d shots: %d differ from measure_all, near ties %d" % (shots, mismatch, ties), flush=True)
+    assert mismatch <= ties
 
 
 def test_sampled_bits_match_oracle_measure_all():

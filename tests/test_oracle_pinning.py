@@ -318,6 +318,26 @@ def test_run_shots_equals_measure_all_on_copies():  # simulator.rs:107-164
         assert list(bits[s]) == list(res)
 
 
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_measure_all_shots_equals_measure_all_on_copies(dtype):
+    """The memoised many-shot measure_all gives every shot the bits that
+    measure_all on its own copy of the state gives (state.rs:329-335,
+    mixed_precision.rs:473-515 for complex-f32), including shots whose draws
+    sit exactly on a threshold."""
+    n, shots = 9, 400
+    psi = random_state(n, 21, dtype)
+    draws = workloads.uniforms(13, shots * n).reshape(shots, n)
+    a = psi.copy()
+    _, p0 = orc.measure(a, n, 0, 0.0)
+    draws[:5, 0] = p0  # draw == p0 measures 1 (state.rs:302)
+    bits = orc.measure_all_shots(psi, n, draws)
+    for s in range(shots):
+        a = psi.copy()
+        r, _ = orc.measure_all(a, n, draws[s])
+        assert list(bits[s]) == [int(x) for x in r], s
+    assert np.all(bits[:5, 0] == 1)
+
+
 def test_run_shots_with_midcircuit_measure():
     n = 3
     c = QuantumCircuit(n).h(0).cnot(0, 1).measure(0).h(2).measure(2)
