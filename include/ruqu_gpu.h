@@ -265,6 +265,16 @@ int rq_expectation(const rq_state* s, const rq_pauli_factor* ops, size_t nops, d
 /* expectation_hamiltonian: term k = coeffs[k] * ops[offsets[k] .. offsets[k+1]). */
 int rq_expectation_hamiltonian(const rq_state* s, const double* coeffs, const uint64_t* offsets,
                                const rq_pauli_factor* ops, size_t nterms, double* out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, code that forbids hipMalloc): the same result
+ * as rq_expectation_hamiltonian, bit for bit, with its reduction partials in a
+ * fixed device array of the library (32 x 2049 doubles, 512.5 KiB per GPU,
+ * held until the process ends) instead of the state's grow-on-demand buffer.
+ * Any state size and term count fits (terms go 32 per launch, partials are
+ * capped at 2048 blocks).  Calls that overlap on one GPU share that array and
+ * race; calls on one stream, or one after another, are safe. */
+int rq_expectation_hamiltonian_noalloc(const rq_state* s, const double* coeffs, const uint64_t* offsets,
+                                       const rq_pauli_factor* ops, size_t nterms, double* out);
 /* <Z_q> for every qubit in one pass (out has num_qubits entries). */
 int rq_expectation_z_all(const rq_state* s, double* out);
 int rq_normalize(rq_state* s);                                  /* state.rs:415-423 */

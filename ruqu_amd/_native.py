@@ -98,6 +98,7 @@ SIGNATURES = {
     "rq_probability_of_qubit": (_i, [_vp, _u32, _pd]),
     "rq_expectation": (_i, [_vp, _vp, _sz, _pd]),
     "rq_expectation_hamiltonian": (_i, [_vp, _vp, _vp, _vp, _sz, _pd]),
+    "rq_expectation_hamiltonian_noalloc": (_i, [_vp, _vp, _vp, _vp, _sz, _pd]),
     "rq_expectation_z_all": (_i, [_vp, _vp]),
     "rq_normalize": (_i, [_vp]),
     "rq_fidelity": (_i, [_vp, _vp, _pd]),

@@ -99,6 +99,9 @@ struct PauliCanon {
   uint32_t ny;
 };
 int reduce_pauli(State& s, const PauliCanon* terms, size_t nterms, double* out);
+// Allocation-free: one batch of 1..32 terms, partials in a fixed device
+// scratch array; adds coeffs[k] * <P_k> to *acc in term order.
+int reduce_hamiltonian_noalloc(State& s, const double* coeffs, const PauliCanon* terms, size_t nterms, double* acc);
 int reduce_z_all(State& s, double* out);
 int reduce_inner(const State& a, const State& b, double* re, double* im);
 int launch_probabilities(State& s, double* d_out);

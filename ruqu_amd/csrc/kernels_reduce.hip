@@ -5,6 +5,7 @@
 // per block, then one block folds the partials in a fixed order, so results
 // are deterministic run to run.
 #include <cstring>
+#include <mutex>
 
 #include "internal.h"
 
@@ -332,6 +333,36 @@ int read_red(State& s, size_t offset, size_t count, double* out) {
   return RQ_OK;
 }
 
+// Fixed device scratch for the allocation-free Hamiltonian expectation: the
+// per-block partials of one launch of k_pauli (up to MAX_TERMS_PER_LAUNCH
+// terms x MAX_RED_BLOCKS blocks) and the folded per-term values.  Each GPU
+// holds its own copy for the life of the process; every call on a GPU shares
+// it, so calls that overlap on one GPU race.  Every slot a call reads was
+// written earlier in the same call, so nothing needs clearing.
+constexpr size_t NOALLOC_DOUBLES = (size_t)MAX_TERMS_PER_LAUNCH * (MAX_RED_BLOCKS + 1);
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+// Device address of `scratch` on each GPU, looked up once per device with that
+// device current (the host-side symbol is only a stand-in for it).
+constexpr int NOALLOC_MAX_DEVICES = 64;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+int scratch_for_current_device(double** out) {
+  int dev = -1;
+  RQ_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(RQ_ERR_DEVICE, "device index out of range for the fixed scratch");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(e == hipSuccess ? hipErrorInvalidSymbol : e, "scratch symbol address");
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return RQ_OK;
+}
+
 }  // namespace
 
 int reduce_masked_normsq(State& s, uint64_t mask, uint64_t want, double* out) {
@@ -381,6 +412,32 @@ int reduce_pauli(State& s, const PauliCanon* terms, size_t nterms, double* out) 
     rc = read_red(s, 0, cnt, out + t0);
     if (rc) return rc;
   }
+  return RQ_OK;
+}
+
+int reduce_hamiltonian_noalloc(State& s, const double* coeffs, const PauliCanon* terms, size_t nterms, double* acc) {
+  const unsigned nb = red_blocks(s.n);
+  if ((size_t)MAX_TERMS_PER_LAUNCH * (nb + 1) > NOALLOC_DOUBLES)  // nb <= MAX_RED_BLOCKS: never taken
+    return fail(RQ_ERR_ARGUMENT, "state too large for the fixed scratch");
+  double* base = nullptr;
+  int rc = scratch_for_current_device(&base);
+  if (rc) return rc;
+  const int cnt = (int)nterms;
+  if (cnt < 1 || cnt > MAX_TERMS_PER_LAUNCH) return fail(RQ_ERR_ARGUMENT, "term batch out of range");
+  PauliBatch b{};
+  for (int k = 0; k < cnt; ++k) b.t[k] = PauliDev{terms[k].x, terms[k].zy, terms[k].ny, 0};
+  double* part = base + MAX_TERMS_PER_LAUNCH;
+  if (s.prec == RQ_C64)
+    hipLaunchKernelGGL(k_pauli<double2>, dim3(nb), dim3(BLOCK), 0, s.stream, (const double2*)s.amps, s.n, b, cnt,
+                       part);
+  else
+    hipLaunchKernelGGL(k_pauli<float2>, dim3(nb), dim3(BLOCK), 0, s.stream, (const float2*)s.amps, s.n, b, cnt,
+                       part);
+  hipLaunchKernelGGL(k_fold, dim3(cnt), dim3(BLOCK), 0, s.stream, part, (int)nb, base);
+  RQ_HIP(hipGetLastError());
+  RQ_HIP(hipMemcpyAsync(s.host_red, base, cnt * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+  RQ_HIP(hipStreamSynchronize(s.stream));
+  for (int k = 0; k < cnt; ++k) *acc += coeffs[k] * s.host_red[k];  // state.rs:403-408, in term order
   return RQ_OK;
 }
 

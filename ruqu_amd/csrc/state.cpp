@@ -789,6 +789,28 @@ int rq_expectation_hamiltonian(const rq_state* h, const double* coeffs, const ui
   return RQ_OK;
 }
 
+int rq_expectation_hamiltonian_noalloc(const rq_state* h, const double* coeffs, const uint64_t* offsets,
+                                       const rq_pauli_factor* ops, size_t nterms, double* out) {
+  State* s = (State*)h;
+  if (!s || !out || (nterms && (!coeffs || !offsets))) return fail(RQ_ERR_ARGUMENT, "null pointer");
+  constexpr size_t BATCH = 32;  // terms per launch (the scratch holds one launch)
+  PauliCanon cs[BATCH];
+  double acc = 0.0;  // state.rs:403-408: sequential sum of c_k * <P_k>
+  for (size_t t0 = 0; t0 < nterms; t0 += BATCH) {
+    const size_t cnt = nterms - t0 < BATCH ? nterms - t0 : BATCH;
+    for (size_t k = 0; k < cnt; ++k) {
+      const size_t t = t0 + k;
+      if (offsets[t + 1] < offsets[t]) return fail(RQ_ERR_ARGUMENT, "offsets must be non-decreasing");
+      int rc = canon_pauli(*s, ops + offsets[t], offsets[t + 1] - offsets[t], &cs[k]);
+      if (rc) return rc;
+    }
+    int rc = reduce_hamiltonian_noalloc(*s, coeffs + t0, cs, cnt, &acc);
+    if (rc) return rc;
+  }
+  *out = acc;
+  return RQ_OK;
+}
+
 int rq_expectation_z_all(const rq_state* h, double* out) {
   State* s = (State*)h;
   if (!s || !out) return fail(RQ_ERR_ARGUMENT, "null pointer");
