@@ -153,6 +153,7 @@ int run_segment(State& s, const rq_gate* g, size_t ng, FusionStats* st) {
   std::vector<float> host(per * blocks.size());
   for (size_t i = 0; i < blocks.size(); ++i) embed(blocks[i], host.data() + i * per);
   const size_t bytes = host.size() * sizeof(float);
+  invalidate_fused_program(s);  // s.prog is about to hold the dense blocks
   if (s.prog_cap < bytes) {
     if (s.prog) RQ_HIP(hipFree(s.prog));
     s.prog = nullptr;

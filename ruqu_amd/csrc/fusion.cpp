@@ -343,15 +343,13 @@ int run_fused(State& s, const rq_gate* gates, size_t ngates, const double* draws
           }
         }
       } else {
-        FusedPlan plan;
-        int rc = plan_fused(s.nq, gates + k, e - k, fusion_config(), &plan);
+        uint64_t unitary = 0, stages = 0;
+        int rc = run_fused_segment(s, gates + k, e - k, &unitary, &stages);
         if (rc) return rc;
-        rc = launch_fused_plan(s, plan);
-        if (rc) return rc;
-        st->unitary += plan.unitary;
-        st->passes += plan.stages.size();
-        st->hbm_bytes += pass_bytes * (double)plan.stages.size();
-        s.gate_count += plan.unitary;
+        st->unitary += unitary;
+        st->passes += stages;
+        st->hbm_bytes += pass_bytes * (double)stages;
+        s.gate_count += unitary;
       }
     }
     if (e < ngates) {

@@ -99,6 +99,9 @@ int plan_fused(uint32_t nq, const rq_gate* gates, size_t ngates, const FusionCon
 
 // Device launch of a planned segment (kernels_fused.hip).
 int launch_fused_plan(State& s, const FusedPlan& plan);
+// Plan (or reuse the state's cached program for the same gates and knobs),
+// upload and launch one segment without Measure/Reset gates.
+int run_fused_segment(State& s, const rq_gate* gates, size_t ngates, uint64_t* unitary, uint64_t* stages);
 
 // Host execution of the program the kernel receives for a plan (packed op
 // records, folded gates, matrix-core phases) with the kernel's data mapping

@@ -34,6 +34,8 @@ inline dim3 grid_blocks(uint64_t blocks) {
 }
 __device__ __forceinline__ uint64_t flat_block() { return (uint64_t)blockIdx.y * gridDim.x + blockIdx.x; }
 
+struct FusedProgram;  // the fused program cached in State::prog (kernels_fused.hip)
+
 // Device-resident state vector.
 struct State {
   uint32_t nq = 0;
@@ -46,8 +48,9 @@ struct State {
   double* host_red = nullptr;  // pinned host mirror for small results
   std::vector<rq_measurement> record;
   uint64_t gate_count = 0;   // for precision_error_bound
-  void* prog = nullptr;      // device copy of the fused program
+  void* prog = nullptr;      // device copy of the fused / dense program
   size_t prog_cap = 0;
+  FusedProgram* fused = nullptr;  // what `prog` holds when it is a reusable fused program
   size_t amp_bytes() const { return prec == RQ_C64 ? 16 : 8; }
   size_t bytes() const { return n * amp_bytes(); }
 };
@@ -56,6 +59,11 @@ struct State {
 int apply_gate_impl(State& s, const rq_gate& g, double draw, rq_measurement* out, int* has_out);
 // Algorithmic bytes of the exact kernel for a gate kind.
 double gate_bytes(const State& s, int kind);
+
+// The fused program cached in s.prog: invalidate before writing s.prog
+// otherwise; release when the state is destroyed.
+void invalidate_fused_program(State& s);
+void release_fused_program(State& s);
 
 // Ensure at least `doubles` of reduction scratch on the state.
 int ensure_red(State& s, size_t doubles);

@@ -930,8 +930,26 @@ int pack_ops(const FusedPlan& plan, int mf_min, std::vector<POp>& out, std::vect
   return RQ_OK;
 }
 
+}  // namespace
+
+// The packed program of the last fused segment a state ran, kept in s.prog:
+// the exact gates and knobs it was planned from, and where its parts sit.
+struct FusedProgram {
+  bool valid = false;
+  std::vector<rq_gate> gates;
+  uint32_t nq = 0;
+  int prec = -1;
+  FusionConfig cfg;
+  std::vector<FStage> stages;
+  size_t o_st = 0, o_ph = 0, o_op = 0, o_pool = 0;
+  uint64_t unitary = 0;
+};
+
+namespace {
+
+// Packs a plan into s.prog (grown when too small) and records the layout in fp.
 template <class T>
-int launch_plan_t(State& s, const FusedPlan& plan) {
+int upload_plan_t(State& s, const FusedPlan& plan, FusedProgram* fp) {
   std::vector<POp> pops;
   std::vector<FPhase> phases;
   std::vector<rq_c64> poolx;
@@ -960,19 +978,46 @@ int launch_plan_t(State& s, const FusedPlan& plan) {
   // The host buffer is pageable: wait for the copy before it goes out of scope.
   RQ_HIP(hipMemcpyAsync(s.prog, host.data(), total, hipMemcpyHostToDevice, s.stream));
   RQ_HIP(hipStreamSynchronize(s.stream));
-  unsigned char* base = reinterpret_cast<unsigned char*>(s.prog);
-  for (size_t i = 0; i < nst; ++i) {
-    const FStage& st = plan.stages[i];
+  fp->stages = plan.stages;
+  fp->o_st = o_st;
+  fp->o_ph = o_ph;
+  fp->o_op = o_op;
+  fp->o_pool = o_pool;
+  fp->unitary = plan.unitary;
+  return RQ_OK;
+}
+
+// One launch per stage of the program in s.prog (asynchronous on s.stream).
+template <class T>
+int launch_program_t(State& s, const FusedProgram& fp) {
+  const unsigned char* base = reinterpret_cast<const unsigned char*>(s.prog);
+  for (size_t i = 0; i < fp.stages.size(); ++i) {
+    const FStage& st = fp.stages[i];
     const unsigned threads = 1u << (st.m - FUSE_R_BITS);
     const uint64_t tiles = 1ull << st.nouter;
     hipLaunchKernelGGL(k_fused_stage<T>, dim3((unsigned)tiles), dim3(threads), 0, s.stream, (T*)s.amps,
-                       reinterpret_cast<const FStage*>(base + o_st + i * sizeof(FStage)),
-                       reinterpret_cast<const FPhase*>(base + o_ph), reinterpret_cast<const POp*>(base + o_op),
-                       reinterpret_cast<const Cx<T>*>(base + o_pool));
+                       reinterpret_cast<const FStage*>(base + fp.o_st + i * sizeof(FStage)),
+                       reinterpret_cast<const FPhase*>(base + fp.o_ph), reinterpret_cast<const POp*>(base + fp.o_op),
+                       reinterpret_cast<const Cx<T>*>(base + fp.o_pool));
   }
   RQ_HIP(hipGetLastError());
   return RQ_OK;
 }
+
+int check_shapes(const FusedPlan& plan) {
+  for (const FStage& st : plan.stages)
+    if (st.m < FUSE_R_BITS || st.m > FUSE_MAX_M || st.nouter > 31)
+      return fail(RQ_ERR_ARGUMENT, "fused stage shape out of range");
+  return RQ_OK;
+}
+
+bool same_config(const FusionConfig& a, const FusionConfig& b) {
+  return a.m == b.m && a.c == b.c && a.mf == b.mf && a.seeds == b.seeds;
+}
+
+}  // namespace
+
+namespace {
 
 
 // ---- host execution of the packed program -----------------------------------
@@ -1142,11 +1187,50 @@ int packed_info(const FusedPlan& plan, int precision, int mf_min, uint64_t* reco
 
 int launch_fused_plan(State& s, const FusedPlan& plan) {
   if (plan.stages.empty()) return RQ_OK;
-  for (const FStage& st : plan.stages)
-    if (st.m < FUSE_R_BITS || st.m > FUSE_MAX_M || st.nouter > 31)
-      return fail(RQ_ERR_ARGUMENT, "fused stage shape out of range");
-  if (s.prec == RQ_C64) return launch_plan_t<double>(s, plan);
-  return launch_plan_t<float>(s, plan);
+  if (int rc = check_shapes(plan)) return rc;
+  invalidate_fused_program(s);  // s.prog is about to hold another program
+  FusedProgram fp;
+  int rc = s.prec == RQ_C64 ? upload_plan_t<double>(s, plan, &fp) : upload_plan_t<float>(s, plan, &fp);
+  if (rc) return rc;
+  return s.prec == RQ_C64 ? launch_program_t<double>(s, fp) : launch_program_t<float>(s, fp);
+}
+
+// Plans (unless the state's cached program was planned from the same gates,
+// size, precision and knobs), uploads, and launches one fused segment.
+int run_fused_segment(State& s, const rq_gate* gates, size_t ngates, uint64_t* unitary, uint64_t* stages) {
+  const FusionConfig cfg = fusion_config();
+  FusedProgram* fp = s.fused;
+  const bool hit = fp && fp->valid && fp->nq == s.nq && fp->prec == s.prec && same_config(fp->cfg, cfg) &&
+                   fp->gates.size() == ngates && std::memcmp(fp->gates.data(), gates, ngates * sizeof(rq_gate)) == 0;
+  if (!hit) {
+    FusedPlan plan;
+    int rc = plan_fused(s.nq, gates, ngates, cfg, &plan);
+    if (rc) return rc;
+    if ((rc = check_shapes(plan))) return rc;
+    if (!fp) fp = s.fused = new FusedProgram();
+    fp->valid = false;
+    rc = s.prec == RQ_C64 ? upload_plan_t<double>(s, plan, fp) : upload_plan_t<float>(s, plan, fp);
+    if (rc) return rc;
+    fp->gates.assign(gates, gates + ngates);
+    fp->nq = s.nq;
+    fp->prec = s.prec;
+    fp->cfg = cfg;
+    fp->valid = true;
+  }
+  int rc = s.prec == RQ_C64 ? launch_program_t<double>(s, *fp) : launch_program_t<float>(s, *fp);
+  if (rc) return rc;
+  *unitary = fp->unitary;
+  *stages = fp->stages.size();
+  return RQ_OK;
+}
+
+void invalidate_fused_program(State& s) {
+  if (s.fused) s.fused->valid = false;
+}
+
+void release_fused_program(State& s) {
+  delete s.fused;
+  s.fused = nullptr;
 }
 
 }  // namespace rq

@@ -457,6 +457,7 @@ void rq_state_destroy(rq_state* h) {
   if (s->amps) (void)hipFree(s->amps);
   if (s->red) (void)hipFree(s->red);
   if (s->prog) (void)hipFree(s->prog);
+  release_fused_program(*s);
   if (s->host_red) (void)hipHostFree(s->host_red);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;

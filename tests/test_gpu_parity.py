@@ -206,6 +206,48 @@ def test_32_qubit_state():
     exact.state.close()
 
 
+@pytest.mark.parametrize("prec", ["c64", "c32"])
+def test_fused_program_reuse(prec):
+    """The fused runner keeps the uploaded program of the last segment and
+    reuses it when the same gates run again with the same knobs: repeated,
+    interleaved (another circuit, the dense path, a changed planner knob) and
+    continued runs must all equal fresh runs bit for bit."""
+    from ruqu_amd._native import lib
+    n = 14
+    flush("fused program reuse %s" % prec)
+    a = workloads.random_circuit(n, 6, 11)
+    b = workloads.random_circuit(n, 6, 12)
+
+    def fresh(c):
+        r = Simulator.run_with_config(c, SimConfig(precision=prec, fusion=True))
+        v = r.state.state_vector()
+        r.state.close()
+        return v
+
+    want_a, want_b = fresh(a), fresh(b)
+    r = Simulator.run_with_config(a, SimConfig(precision=prec, fusion=True))
+    st = r.state
+    assert np.array_equal(st.state_vector(), want_a)
+    for c, want in ((a, want_a), (a, want_a), (b, want_b), (a, want_a)):
+        Simulator.run_with_config(c, SimConfig(precision=prec, fusion=True), state=st)
+        assert np.array_equal(st.state_vector(), want)
+    if prec == "c32":
+        Simulator.run_with_config(b, SimConfig(precision=prec, fusion=2), state=st)  # dense blocks use the buffer
+        Simulator.run_with_config(a, SimConfig(precision=prec, fusion=True), state=st)
+        assert np.array_equal(st.state_vector(), want_a)
+    old = lib().rq_tune_get(5)
+    try:
+        assert lib().rq_tune_set(5, 0) == 0  # another planner knob: a new plan
+        want0 = fresh(a)
+        Simulator.run_with_config(a, SimConfig(precision=prec, fusion=True), state=st)
+        assert np.array_equal(st.state_vector(), want0)
+    finally:
+        lib().rq_tune_set(5, old)
+    Simulator.run_with_config(a, SimConfig(precision=prec, fusion=True), state=st)
+    assert np.array_equal(st.state_vector(), want_a)
+    st.close()
+
+
 def test_config1_20q_c64():
     """20-qubit random circuit, depth 20, complex-f64: all amplitudes, <Z_i>
     for every qubit and 10^4 measured shots."""
