@@ -407,6 +407,12 @@ int rq_fusion_emulate(uint32_t num_qubits, int precision, const rq_gate* gates, 
  * composed unitaries they select from. */
 int rq_fusion_packed_info(uint32_t num_qubits, int precision, const rq_gate* gates, size_t ngates, int m, int c,
                           uint64_t* records, uint64_t* mf_phases, uint64_t* mf_unitaries);
+/* Host-only: that program's records by kind (counts[9]): 0 dense 2x2, 1 dense
+ * 4x4, 2 one-bit diagonal, 3 two-bit diagonal, 4 CNOT, 5 CZ, 6 padding,
+ * 7 factored 2x2 (unit-diagonal or unit-antidiagonal form), 8 pending-diagonal
+ * tables over the register bits. */
+int rq_fusion_packed_kinds(uint32_t num_qubits, int precision, const rq_gate* gates, size_t ngates, int m, int c,
+                           uint64_t* counts);
 
 /* Process-wide tuning knobs (no reference counterpart; set by the autotuner):
  * key 0 = streaming kernels' amplitude groups per thread (1, 2, 4, 8),

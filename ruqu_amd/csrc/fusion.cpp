@@ -424,6 +424,19 @@ int rq_fusion_packed_info(uint32_t nq, int precision, const rq_gate* gates, size
   return rq::packed_info(plan, precision, rq::fusion_config().mf, records, mf_phases, mf_unitaries);
 }
 
+// Record counts of that program by kind (counts has 9 entries).  Host only.
+int rq_fusion_packed_kinds(uint32_t nq, int precision, const rq_gate* gates, size_t ngates, int m, int c,
+                           uint64_t* counts) {
+  if (!counts) return rq::fail(RQ_ERR_ARGUMENT, "null pointer");
+  rq::FusedPlan plan;
+  rq::FusionConfig cfg = rq::fusion_config();  // the current knobs, with this tile shape
+  cfg.m = m;
+  cfg.c = c;
+  int rc = rq::plan_fused(nq, gates, ngates, cfg, &plan);
+  if (rc) return rc;
+  return rq::packed_kinds(plan, precision, rq::fusion_config().mf, counts);
+}
+
 // Sets the tile size (m qubits) and the count of low qubits always kept in a
 // tile (c) used by the fused schedule of rq_run_circuit.
 int rq_fusion_configure(int m, int c) {

@@ -110,6 +110,10 @@ int run_fused_segment(State& s, const rq_gate* gates, size_t ngates, uint64_t* u
 int emulate_packed(const FusedPlan& plan, int precision, int mf_min, void* amps);
 // Size of the packed program: op records, matrix-core phases and their
 // composed unitaries.
+// Records of the packed program by kind: counts[0..7] per record kind (2x2,
+// 4x4, 1-bit diagonal, 2-bit diagonal, CNOT, CZ, padding, factored 2x2),
+// counts[8] the pending-diagonal tables (counted apart from kind 2).
+int packed_kinds(const FusedPlan& plan, int precision, int mf_min, uint64_t* counts);
 int packed_info(const FusedPlan& plan, int precision, int mf_min, uint64_t* records, uint64_t* mf_phases,
                 uint64_t* mf_unitaries);
 

@@ -239,6 +239,48 @@ __device__ __forceinline__ void dall(const Amps<T>& x, Amps<T>& y, Cx<T> f) {
 #pragma unroll
   for (int k = 0; k < FUSE_R; ++k) y[k] = cmul(f, x[k]);
 }
+template <class T>
+__device__ __forceinline__ void dtab(const Amps<T>& x, Amps<T>& y, const Cx<T>* __restrict__ mp) {
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) y[k] = cmul(mp[k], x[k]);
+}
+
+// a + c b with one complex coefficient: 4 FMAs.
+template <class T>
+__device__ __forceinline__ Cx<T> axpy1(Cx<T> a, T cr, T ci, Cx<T> b) {
+  return Cx<T>{__builtin_fma(-ci, b.im, __builtin_fma(cr, b.re, a.re)),
+               __builtin_fma(ci, b.re, __builtin_fma(cr, b.im, a.im))};
+}
+template <>
+__device__ __forceinline__ Cx<float> axpy1(Cx<float> a, float cr, float ci, Cx<float> b) {
+  return Cx<float>{__builtin_fmaf(-ci, b.im, __builtin_fmaf(cr, b.re, a.re)),
+                   __builtin_fmaf(ci, b.re, __builtin_fmaf(cr, b.im, a.im))};
+}
+
+// Factored 2x2 on register bit B: [[1, r], [s, 1]] (8 FMAs per pair instead
+// of 16) or, with ANTI, [[r, 1], [1, s]].
+template <int B, class T>
+__device__ __forceinline__ void u1f(const Amps<T>& x, Amps<T>& y, const Cx<T>* __restrict__ mp) {
+  const T rr = mp[0].re, ri = mp[0].im, sr = mp[1].re, si = mp[1].im;
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) {
+    if (k & (1 << B)) continue;
+    const int k1 = k | (1 << B);
+    y[k] = axpy1(x[k], rr, ri, x[k1]);
+    y[k1] = axpy1(x[k1], sr, si, x[k]);
+  }
+}
+template <int B, class T>
+__device__ __forceinline__ void u1f_anti(const Amps<T>& x, Amps<T>& y, const Cx<T>* __restrict__ mp) {
+  const T rr = mp[0].re, ri = mp[0].im, sr = mp[1].re, si = mp[1].im;
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) {
+    if (k & (1 << B)) continue;
+    const int k1 = k | (1 << B);
+    y[k] = axpy1(x[k1], rr, ri, x[k]);
+    y[k1] = axpy1(x[k], sr, si, x[k1]);
+  }
+}
 
 // Packed op record (built from FOp at launch, one 8-byte scalar load per op):
 //   x: kind:3 | t0:2 | t1:2 | cloc:2 | cpos:5 | dloc0:2 | dpos0:6 | dloc1:2 | dpos1:6
@@ -250,6 +292,12 @@ struct POp {
 };
 constexpr int PL_NONE = 0, PL_REG = LOC_REG + 1, PL_THR = LOC_THR + 1, PL_OUT = LOC_OUT + 1;
 constexpr int POP_NOP = 6;  // kind of the padding record: copies the amplitudes
+// Factored 2x2 on register bit t0 (pool: r, s): t1 = 0 the unit-diagonal form
+// [[1, r], [s, 1]], t1 = 1 the unit-antidiagonal form [[r, 1], [1, s]].  The
+// diagonal factor the host split off stays in the phase's pending diagonal.
+constexpr int POP_U1F = 7;
+// A D1 record with location PL_NONE: the pending diagonal over the 4 register
+// bits, one factor per register index (pool: 16 entries).
 
 // Value of a non-register location bit for this thread / tile.
 __device__ __forceinline__ uint32_t loc_val(int loc, int pos, uint32_t tid) {
@@ -320,8 +368,17 @@ __device__ __forceinline__ void apply_one(const Amps<T>& x, Amps<T>& y, uint32_t
     case FOP_D1:
       if (dl0 == PL_REG) {
         RQ_P1(dsel_r, dp0, x, y, mp[0], mp[1])
+      } else if (dl0 == PL_NONE) {
+        dtab(x, y, mp);
       } else {
         dall(x, y, mp[loc_val(dl0, dp0, tid)]);
+      }
+      break;
+    case POP_U1F:  // cloc/cpos: a tile-id bit that toggles the form
+      if (t1 ^ (cl == PL_OUT ? loc_val(cl, cp, tid) : 0u)) {
+        RQ_P1(u1f_anti, t0, x, y, mp)
+      } else {
+        RQ_P1(u1f, t0, x, y, mp)
       }
       break;
     case POP_NOP:
@@ -578,12 +635,13 @@ __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FS
 
 // ---- matrix-core phases: host side ------------------------------------------
 
-// VALU instructions per wave of one op (16 amplitudes per thread): a 2x2 is
-// 8 pairs x 16 FP32 ops, a 4x4 twice that; diagonals one complex product
-// per amplitude; permutations and sign flips one move per word.
+// VALU instructions per wave of one op (16 amplitudes per thread): a
+// factored 2x2 is 8 pairs x 8 FMAs, a 4x4 8 x 16 x 4 / 2; diagonals one
+// complex product per amplitude; permutations and sign flips one move per
+// word.  A phase adds one 64-instruction pending-diagonal table.
 int op_cost(const FOp& op) {
   switch (op.kind) {
-    case FOP_U1: return 128;
+    case FOP_U1: return 64;
     case FOP_U2: return 256;
     case FOP_D1:
     case FOP_D2: return 64;
@@ -742,6 +800,105 @@ void mf_operands(const rq_c64 (*U)[FUSE_R], float* w) {
   }
 }
 
+// Pending diagonal of a VALU phase over its 4 register bits: entry k is
+// g[k] * d[0][k_0] * d[1][k_1] * d[2][k_2] * d[3][k_3].  The per-bit factors d
+// come from the factored 2x2 ops and from 1-qubit diagonals; g holds what
+// couples bits (CZ signs, 2-qubit diagonals, factors carried through a
+// CNOT's permutation).  g is updated only by multiplications whose factor is
+// a function of index bits and by permutations, so two entries that differ in
+// bits g does not depend on hold bit-identical values: `==` tests
+// independence exactly.
+struct PendingDiag {
+  rq_c64 d[FUSE_R_BITS][2];
+  rq_c64 g[FUSE_R];
+  PendingDiag() {
+    for (auto& b : d) b[0] = b[1] = rq_c64{1.0, 0.0};
+    for (auto& x : g) x = rq_c64{1.0, 0.0};
+  }
+  static rq_c64 mul(rq_c64 a, rq_c64 b) { return rq_c64{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+  static bool eq(rq_c64 a, rq_c64 b) { return a.re == b.re && a.im == b.im; }
+  bool identity() const {
+    const rq_c64 one{1.0, 0.0};
+    for (const auto& b : d)
+      if (!eq(b[0], one) || !eq(b[1], one)) return false;
+    for (const auto& x : g)
+      if (!eq(x, one)) return false;
+    return true;
+  }
+  rq_c64 entry(int k) const {
+    rq_c64 v = g[k];
+    for (int b = 0; b < FUSE_R_BITS; ++b) v = mul(v, d[b][(k >> b) & 1]);
+    return v;
+  }
+  bool free_of(int b) const {  // g does not depend on bit b
+    for (int k = 0; k < FUSE_R; ++k)
+      if (!eq(g[k], g[k ^ (1 << b)])) return false;
+    return true;
+  }
+  bool flip_invariant(int b) const { return free_of(b) && eq(d[b][0], d[b][1]); }
+  void clear_bit(int b) { d[b][0] = d[b][1] = rq_c64{1.0, 0.0}; }
+  void set_bit(int b, const rq_c64 e[2]) { d[b][0] = e[0]; d[b][1] = e[1]; }
+  // M diag(d[b]): the factor applied before M moves into its columns.
+  void fold_columns(int b, rq_c64 m[4]) const {
+    m[0] = mul(m[0], d[b][0]); m[2] = mul(m[2], d[b][0]);
+    m[1] = mul(m[1], d[b][1]); m[3] = mul(m[3], d[b][1]);
+  }
+  // 4x4 on (b0 = row-high bit, b1): column c = 2 bit_b0 + bit_b1.
+  void fold_columns4(int b0, int b1, rq_c64 m[16]) const {
+    for (int r = 0; r < 4; ++r)
+      for (int c = 0; c < 4; ++c) m[r * 4 + c] = mul(mul(m[r * 4 + c], d[b0][c >> 1]), d[b1][c & 1]);
+  }
+  // CNOT(c -> t) after the diagonal: D then X_t^c equals X_t^c then D o pi.
+  void cx_reg(int c, int t) {
+    for (int k = 0; k < FUSE_R; ++k) g[k] = mul(g[k], d[t][(k >> t) & 1]);
+    clear_bit(t);
+    rq_c64 h[FUSE_R];
+    for (int k = 0; k < FUSE_R; ++k) h[k] = g[k ^ (((k >> c) & 1) << t)];
+    for (int k = 0; k < FUSE_R; ++k) g[k] = h[k];
+  }
+  void cx_all(int t) {
+    std::swap(d[t][0], d[t][1]);
+    rq_c64 h[FUSE_R];
+    for (int k = 0; k < FUSE_R; ++k) h[k] = g[k ^ (1 << t)];
+    for (int k = 0; k < FUSE_R; ++k) g[k] = h[k];
+  }
+  void cz(int p0, int p1) {
+    for (int k = 0; k < FUSE_R; ++k)
+      if ((k >> p0) & (k >> p1) & 1) g[k] = rq_c64{-g[k].re, -g[k].im};
+  }
+  void d1(int b, const rq_c64* f) { d[b][0] = mul(d[b][0], f[0]); d[b][1] = mul(d[b][1], f[1]); }
+  void d2(int p0, int p1, const rq_c64* f) {
+    for (int k = 0; k < FUSE_R; ++k) g[k] = mul(g[k], f[2 * ((k >> p0) & 1) + ((k >> p1) & 1)]);
+  }
+};
+
+// M = diag(e0, e1) F with F = [[1, r], [s, 1]] (returns 0) or [[r, 1], [1, s]]
+// (returns 1), the form whose ratios are the smaller (|r|, |s| <= 1 for a
+// unitary M); -1 when neither exists.
+int factor_2x2(const rq_c64 m[4], rq_c64 rs[2], rq_c64 e[2]) {
+  auto mag2 = [](rq_c64 a) { return a.re * a.re + a.im * a.im; };
+  auto div = [&](rq_c64 a, rq_c64 b) {
+    const double n = mag2(b);
+    return rq_c64{(a.re * b.re + a.im * b.im) / n, (a.im * b.re - a.re * b.im) / n};
+  };
+  const double dg = mag2(m[0]) * mag2(m[3]), ad = mag2(m[1]) * mag2(m[2]);
+  if (dg >= ad && dg > 0.0) {  // diag(m00, m11) [[1, m01/m00], [m10/m11, 1]]
+    rs[0] = div(m[1], m[0]);
+    rs[1] = div(m[2], m[3]);
+    e[0] = m[0];
+    e[1] = m[3];
+    return 0;
+  }
+  if (ad > 0.0) {  // diag(m01, m10) [[m00/m01, 1], [1, m11/m10]]
+    rs[0] = div(m[0], m[1]);
+    rs[1] = div(m[3], m[2]);
+    e[0] = m[1];
+    e[1] = m[2];
+    return 1;
+  }
+  return -1;
+}
+
 // True when op `x` involves register bit b in any role.
 bool touches_reg_bit(const FOp& x, int b) {
   switch (x.kind) {
@@ -761,13 +918,21 @@ bool touches_reg_bit(const FOp& x, int b) {
 // extended by the folded matrices below.
 //
 // Folding: a CNOT whose target is register bit b and whose control is absent
-// or a tile-id bit c (X_b, or X_b when c = 1), and a CZ of register bit b with
-// a tile-id bit c (Z_b when c = 1), commute with every op that does not
-// involve bit b, and c is constant over the tile.  Such an op is moved to the
-// next op of its phase that involves b when that op is an uncontrolled 2x2 on
-// b (M), and merged into it: M X or M Z is exact (a column swap or sign), and
-// with a control bit the 2x2 takes the pair [M, M X] or [M, M Z] and picks by
-// bit c.  The folded op then costs no register moves at all.
+// or a tile-id bit c (X_b, or X_b when c = 1) commutes with every op that does
+// not involve bit b, and c is constant over the tile.  Such an op is moved to
+// the next op of its phase that involves b when that op is an uncontrolled 2x2
+// on b (M), and merged into it: M X is exact (a column swap), and with a
+// control bit the 2x2 takes the pair [M, M X] and picks by bit c.
+//
+// Factoring: every 2x2 M that reaches the emitter is split as diag(e0, e1) F
+// with F = [[1, r], [s, 1]] or [[r, 1], [1, s]] (factor_2x2), and only F is
+// applied (8 FMAs per amplitude pair instead of 16).  The split-off diagonal
+// joins the phase's pending diagonal (PendingDiag), which also absorbs CZ and
+// diagonal ops on register bits, moves through CNOTs as a permutation, folds
+// into the columns of the next 2x2 / 4x4 on its bits, and is applied as one
+// 16-entry table record where it cannot move further (a CNOT with a per-lane
+// condition on a bit it depends on, an op on a bit it couples to others) and
+// at the end of the phase.
 //
 // Matrix-core phases (mf_min >= 0, complex f32): a phase whose ops cost at
 // least mf_min VALU instructions per wave and whose conditions are register
@@ -796,7 +961,7 @@ int pack_ops(const FusedPlan& plan, int mf_min, std::vector<POp>& out, std::vect
       for (int ph = st.phase0; ph < st.phase0 + st.nphase; ++ph) {
         const FPhase& f = plan.phases[ph];
         if (f.op_end == f.op_begin) continue;
-        int cost = ((f.op_end - f.op_begin) & 1) ? 32 : 0;
+        int cost = 64 + (((f.op_end - f.op_begin) & 1) ? 32 : 0);
         for (int o = f.op_begin; o < f.op_end; ++o) cost += op_cost(plan.ops[o]);
         if (cost < mf_min) continue;
         MfBits b;
@@ -815,10 +980,6 @@ int pack_ops(const FusedPlan& plan, int mf_min, std::vector<POp>& out, std::vect
       const FOp& op = plan.ops[o];
       int b = -1;
       if (op.kind == FOP_CX && !op.ctl_reg && !op.ctl_thr) b = op.t0;
-      if (op.kind == FOP_CZ) {
-        if (op.dloc0 == LOC_REG && op.dloc1 == LOC_OUT) b = op.dpos0;
-        if (op.dloc1 == LOC_REG && op.dloc0 == LOC_OUT) b = op.dpos1;
-      }
       if (b < 0) continue;
       for (int q = o + 1; q < ph.op_end; ++q) {
         const FOp& x = plan.ops[q];
@@ -867,30 +1028,40 @@ int pack_ops(const FusedPlan& plan, int mf_min, std::vector<POp>& out, std::vect
         std::swap(phases[ph].regt[3], phases[ph].thrt[5]);
         continue;
       }
+      PendingDiag pd;
+      auto emit = [&](uint32_t kind, uint32_t t0, uint32_t t1, uint32_t cl, uint32_t cp, uint32_t dl0, uint32_t dp0,
+                      uint32_t dl1, uint32_t dp1, uint32_t mat) {
+        POp w;
+        w.x = kind | (t0 << 3) | (t1 << 5) | (cl << 7) | (cp << 9) | (dl0 << 14) | (dp0 << 16) | (dl1 << 22) |
+              (dp1 << 24);
+        w.y = mat;
+        out.push_back(w);
+      };
+      auto flush = [&]() {  // apply the pending diagonal as one table record
+        if (pd.identity()) return;
+        const uint32_t mat = (uint32_t)pool.size();
+        for (int k = 0; k < FUSE_R; ++k) pool.push_back(pd.entry(k));
+        emit((uint32_t)FOP_D1, 0, 0, 0, 0, (uint32_t)PL_NONE, 0, 0, 0, mat);
+        pd = PendingDiag();
+      };
       for (int o = plan.phases[ph].op_begin; o < plan.phases[ph].op_end; ++o) {
         const FOp& op = plan.ops[o];
         if (fold_into[o] >= 0) continue;  // merged into a later 2x2
         uint32_t cl = 0, cp = 0, dl0 = 0, dp0 = 0, dl1 = 0, dp1 = 0;
         uint32_t mat = (uint32_t)op.mat;
         if (folded_by[o] >= 0) {
-          const FOp& f = plan.ops[folded_by[o]];
+          const FOp& f = plan.ops[folded_by[o]];  // a CNOT
           const rq_c64* m = &plan.pool[op.mat];
-          rq_c64 mf[4];
-          if (f.kind == FOP_CX) {  // M X: columns swapped
-            mf[0] = m[1]; mf[1] = m[0]; mf[2] = m[3]; mf[3] = m[2];
-          } else {  // M Z: second column negated
-            mf[0] = m[0]; mf[1] = rq_c64{-m[1].re, -m[1].im}; mf[2] = m[2]; mf[3] = rq_c64{-m[3].re, -m[3].im};
-          }
-          int cq = -1;  // the condition's global qubit (none: always)
-          if (f.kind == FOP_CX && f.ctl_out) cq = __builtin_ctzll(f.ctl_out);
-          if (f.kind == FOP_CZ) cq = f.dloc0 == LOC_OUT ? f.dpos0 : f.dpos1;
+          const rq_c64 mf[4] = {m[1], m[0], m[3], m[2]};  // M X: columns swapped
+          const int cq = f.ctl_out ? __builtin_ctzll(f.ctl_out) : -1;  // the condition's global qubit (none: always)
           mat = (uint32_t)pool.size();
           if (cq < 0) {
             pool.insert(pool.end(), mf, mf + 4);
           } else {
             const int j = tile_bit(cq);
             if (j < 0 || j >= 32) return fail(RQ_ERR_ARGUMENT, "fused control outside the tile id");
-            pool.insert(pool.end(), m, m + 4);
+            const rq_c64 m0[4] = {m[0], m[1], m[2], m[3]};
+            pool.insert(pool.end(), m0, m0 + 4);
             pool.insert(pool.end(), mf, mf + 4);
             cl = PL_OUT;
             cp = (uint32_t)j;
@@ -917,12 +1088,77 @@ int pack_ops(const FusedPlan& plan, int mf_min, std::vector<POp>& out, std::vect
           if (op.kind != FOP_D1 && !enc(op.dloc1, op.dpos1, &dl1, &dp1))
             return fail(RQ_ERR_ARGUMENT, "fused diagonal bit not located");
         }
-        POp w;
-        w.x = (uint32_t)op.kind | ((uint32_t)op.t0 << 3) | ((uint32_t)op.t1 << 5) | (cl << 7) | (cp << 9) |
-              (dl0 << 14) | (dp0 << 16) | (dl1 << 22) | (dp1 << 24);
-        w.y = mat;
-        out.push_back(w);
+        const int b = op.t0;
+        switch (op.kind) {
+          case FOP_U1: {
+            if (!pd.free_of(b)) flush();
+            // With cl = PL_OUT the pool holds the pair [M, M X] selected by a tile-id bit (a CNOT
+            // controlled by it was folded in), which acts after the pending factor on b: the pair is
+            // [M diag(d), M X diag(d)].  When d[0] = d[1] that is [M', M' X] with M' = M diag(d) =
+            // diag(e) F, and F X is F's other unit form with the same r and s: the tile-id bit toggles
+            // the form.  Otherwise each matrix takes the factor and the pair runs unfactored.
+            if (cl == PL_OUT && !pd.flip_invariant(b)) {
+              const uint32_t nm = (uint32_t)pool.size();
+              for (int v = 0; v < 2; ++v) {
+                rq_c64 m2[4] = {pool[mat + 4 * v], pool[mat + 4 * v + 1], pool[mat + 4 * v + 2], pool[mat + 4 * v + 3]};
+                pd.fold_columns(b, m2);
+                pool.insert(pool.end(), m2, m2 + 4);
+              }
+              pd.clear_bit(b);
+              emit((uint32_t)FOP_U1, (uint32_t)b, 0, cl, cp, 0, 0, 0, 0, nm);
+              break;
+            }
+            rq_c64 m2[4] = {pool[mat], pool[mat + 1], pool[mat + 2], pool[mat + 3]};
+            pd.fold_columns(b, m2);
+            pd.clear_bit(b);
+            rq_c64 rs[2], e[2];
+            int form = factor_2x2(m2, rs, e);
+            const uint32_t nm = (uint32_t)pool.size();
+            if (form < 0) {  // a zero row or column: no unit form, apply the whole matrix (pair)
+              rq_c64 mx[4] = {m2[1], m2[0], m2[3], m2[2]};
+              pool.insert(pool.end(), m2, m2 + 4);
+              pool.insert(pool.end(), mx, mx + 4);
+              emit((uint32_t)FOP_U1, (uint32_t)b, 0, cl, cp, 0, 0, 0, 0, nm);
+            } else {
+              pool.insert(pool.end(), rs, rs + 2);
+              emit((uint32_t)POP_U1F, (uint32_t)b, (uint32_t)form, cl, cp, 0, 0, 0, 0, nm);
+              pd.set_bit(b, e);
+            }
+            break;
+          }
+          case FOP_U2: {
+            if (!pd.free_of(op.t0) || !pd.free_of(op.t1)) flush();
+            rq_c64 m4[16];
+            for (int k = 0; k < 16; ++k) m4[k] = pool[mat + k];
+            pd.fold_columns4(op.t0, op.t1, m4);
+            pd.clear_bit(op.t0);
+            pd.clear_bit(op.t1);
+            const uint32_t nm = (uint32_t)pool.size();
+            pool.insert(pool.end(), m4, m4 + 16);
+            emit((uint32_t)FOP_U2, (uint32_t)op.t0, (uint32_t)op.t1, 0, 0, 0, 0, 0, 0, nm);
+            break;
+          }
+          case FOP_CX:
+            if (cl == PL_REG) pd.cx_reg((int)cp, b);
+            else if (cl == PL_NONE) pd.cx_all(b);
+            else if (!pd.flip_invariant(b)) flush();  // a per-lane / per-tile condition
+            emit((uint32_t)FOP_CX, (uint32_t)b, 0, cl, cp, 0, 0, 0, 0, mat);
+            break;
+          case FOP_CZ:
+            if (dl0 == PL_REG && dl1 == PL_REG) pd.cz((int)dp0, (int)dp1);  // absorbed
+            else emit((uint32_t)FOP_CZ, 0, 0, 0, 0, dl0, dp0, dl1, dp1, mat);
+            break;
+          case FOP_D1:
+            if (dl0 == PL_REG) pd.d1((int)dp0, &pool[mat]);  // absorbed
+            else emit((uint32_t)FOP_D1, 0, 0, 0, 0, dl0, dp0, 0, 0, mat);
+            break;
+          default:  // FOP_D2
+            if (dl0 == PL_REG && dl1 == PL_REG) pd.d2((int)dp0, (int)dp1, &pool[mat]);  // absorbed
+            else emit((uint32_t)FOP_D2, 0, 0, 0, 0, dl0, dp0, dl1, dp1, mat);
+            break;
+        }
       }
+      flush();
       if ((out.size() - (size_t)phases[ph].op_begin) & 1) out.push_back(POp{(uint32_t)POP_NOP, 0u});
       phases[ph].op_end = (int32_t)out.size();
     }
@@ -1015,11 +1251,6 @@ bool same_config(const FusionConfig& a, const FusionConfig& b) {
   return a.m == b.m && a.c == b.c && a.mf == b.mf && a.seeds == b.seeds;
 }
 
-}  // namespace
-
-namespace {
-
-
 // ---- host execution of the packed program -----------------------------------
 
 // One packed record on host amplitudes, with apply_one's semantics.
@@ -1064,7 +1295,20 @@ void host_apply_one(Cx<T> (&x)[FUSE_R], uint32_t w, const Cx<T>* mp, uint32_t ti
     for (int k = 0; k < FUSE_R; ++k)
       if (bit(dl0, dp0, k) & bit(dl1, dp1, k)) y[k] = Cx<T>{-x[k].re, -x[k].im};
   } else if (kind == FOP_D1) {
-    for (int k = 0; k < FUSE_R; ++k) y[k] = mul(mp[bit(dl0, dp0, k)], x[k]);
+    for (int k = 0; k < FUSE_R; ++k) y[k] = mul(dl0 == PL_NONE ? mp[k] : mp[bit(dl0, dp0, k)], x[k]);
+  } else if (kind == POP_U1F) {
+    const int bb = 1 << t0;
+    const int anti = t1 ^ (cl == PL_OUT ? lv(cl, cp) : 0);
+    for (int k = 0; k < FUSE_R; ++k) {
+      if (k & bb) continue;
+      if (anti) {
+        y[k] = add(mul(mp[0], x[k]), x[k | bb]);
+        y[k | bb] = add(x[k], mul(mp[1], x[k | bb]));
+      } else {
+        y[k] = add(x[k], mul(mp[0], x[k | bb]));
+        y[k | bb] = add(mul(mp[1], x[k]), x[k | bb]);
+      }
+    }
   } else if (kind == FOP_D2) {
     for (int k = 0; k < FUSE_R; ++k) y[k] = mul(mp[2 * bit(dl0, dp0, k) + bit(dl1, dp1, k)], x[k]);
   }  // POP_NOP: unchanged
@@ -1182,6 +1426,19 @@ int packed_info(const FusedPlan& plan, int precision, int mf_min, uint64_t* reco
   if (records) *records = pops.size();
   if (mf_phases) *mf_phases = n;
   if (mf_unitaries) *mf_unitaries = u;
+  return RQ_OK;
+}
+
+int packed_kinds(const FusedPlan& plan, int precision, int mf_min, uint64_t* counts) {
+  std::vector<POp> pops;
+  std::vector<FPhase> phases;
+  std::vector<rq_c64> poolx;
+  if (int rc = pack_ops(plan, precision == RQ_C64 ? -1 : mf_min, pops, phases, poolx)) return rc;
+  for (int k = 0; k < 9; ++k) counts[k] = 0;
+  for (const POp& p : pops) {
+    const uint32_t kind = p.x & 7u, dl0 = (p.x >> 14) & 3u;
+    ++counts[(kind == (uint32_t)FOP_D1 && dl0 == (uint32_t)PL_NONE) ? 8 : kind];
+  }
   return RQ_OK;
 }
 

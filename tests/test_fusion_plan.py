@@ -120,3 +120,20 @@ def test_planner_seed_search_knob():
         assert not np.array_equal(states[0], states[128])
     finally:
         lib().rq_tune_set(5, old)
+
+
+def test_factored_records_replace_dense_2x2():
+    """Every 2x2 the emitter meets is applied in factored form (F with a unit
+    diagonal or anti-diagonal, 8 FMAs per pair) with its diagonal part in the
+    phase's pending-diagonal table; the table absorbs register-bit CZ and
+    diagonal ops.  The emulated program equals the oracle either way."""
+    n = 14
+    arr = workloads.random_circuit(n, 10, 21).to_array()
+    want, _, _ = orc.run_circuit(n, arr)
+    cnt = np.zeros(9, dtype=np.uint64)
+    assert lib().rq_fusion_packed_kinds(n, 0, ptr(arr), len(arr), 10, 4, ptr(cnt)) == 0
+    u1, u1f, tables = int(cnt[0]), int(cnt[7]), int(cnt[8])
+    assert u1f > 0 and tables > 0 and u1 < u1f
+    assert max_abs(emulate(n, arr, 10, 4), want) < 1e-12
+    got32 = emulate(n, arr, 10, 4, "c32")
+    assert max_abs(got32, want) < 1e-5
