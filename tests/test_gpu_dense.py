@@ -54,3 +54,26 @@ def test_dense_small_state_falls_back():
     want = orc.run_circuit(6, c.to_array())[0]
     got = Simulator.run_with_config(c, SimConfig(precision="c32", fusion="dense"))
     assert max_abs(got.state.state_vector(), want) <= 1e-5
+
+
+def test_dense_full_size_matches_exact():
+    """The benchmark circuit at full size (30 qubits complex f32, depth 40) as
+    dense 5-qubit blocks against the exact per-gate f32 path, through
+    size-independent checks on the device (FP64): both norms and the
+    normalised fidelity."""
+    from ruqu_amd.state import SimConfig as SC
+    n = 30
+    c = workloads.random_circuit(n, 40, 2)
+    print("dense full size n=%d" % n, flush=True)
+    exact = Simulator.run_with_config(c, SC(precision="c32"))
+    print("  exact path done", flush=True)
+    dense = Simulator.run_with_config(c, SC(precision="c32", fusion="dense"))
+    nd, ne = dense.state.norm_sq(), exact.state.norm_sq()
+    inf = 1.0 - exact.state.fidelity(dense.state) / (nd * ne)
+    print("  norms %.9f %.9f  1-F %.3e  blocks %d for %d gates" % (nd, ne, inf, dense.metrics.passes,
+                                                                 dense.metrics.gate_count), flush=True)
+    assert abs(nd - 1.0) < 1e-4 and abs(ne - 1.0) < 1e-4
+    assert abs(inf) < 1e-9
+    assert dense.metrics.gate_count == exact.metrics.gate_count
+    dense.state.close()
+    exact.state.close()

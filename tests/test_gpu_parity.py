@@ -156,7 +156,7 @@ def test_random_circuit_workload_fused(prec, n):
     assert err < TOL[prec]
 
 
-@pytest.mark.parametrize("n,prec,depth", [(30, "c32", 40), (28, "c64", 20)])
+@pytest.mark.parametrize("n,prec,depth", [(30, "c32", 40), (28, "c64", 20), (32, "c64", 20)])
 def test_full_size_fused_matches_exact(n, prec, depth):
     """The benchmark circuit at full size: the fused schedule against the
     exact per-gate path (bit-identical to the oracle wherever the oracle runs),
