@@ -72,6 +72,7 @@ int ensure_red(State& s, size_t doubles);
 struct StreamTune {
   int unroll = 1;          // amplitude groups per thread: 1, 2, 4 or 8
   bool nontemporal = true; // measured best on MI355X (30-qubit c32 sweep)
+  int xcd_chunk = -1;      // two-stream 1q kernels' block order: -1 per-distance table, 0 linear, else chunk
 };
 StreamTune& stream_tune();
 

@@ -62,6 +62,18 @@ __device__ __forceinline__ void stg(V* p, V v) {
   else *p = v;
 }
 
+// Workgroup order of the two-stream kernels.  Workgroups are dispatched to the
+// 8 XCDs in turn; with chunk C > 0 XCD x takes C consecutive pair blocks at a
+// time (chunk k of XCD x is blocks (8k + x) C .. (8k + x) C + C - 1), so the
+// blocks in flight on one XCD are contiguous in memory.  A bijection on
+// [0, nb) when nb is a multiple of 8 C; the host passes C = 0 otherwise.
+__device__ __forceinline__ uint64_t xcd_block(uint32_t C) {
+  const uint64_t b = flat_block();
+  if (C == 0) return b;
+  const uint64_t x = b & 7u, r = b >> 3;
+  return ((r / C) * 8u + x) * C + (r % C);
+}
+
 // Insert a zero bit at position q of p.
 __device__ __forceinline__ uint64_t ins0(uint64_t p, uint32_t q) {
   uint64_t lo = p & ((1ull << q) - 1);
@@ -71,8 +83,8 @@ __device__ __forceinline__ uint64_t ins0(uint64_t p, uint32_t q) {
 // ---- 1q, f32, q >= 1: each group = two adjacent pairs (float4 each side) --
 template <int U, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_1q_f32_v2(float4* __restrict__ a, uint32_t q, uint64_t ngroups,
-                                                     M2<float> m) {
-  const uint64_t g0 = flat_block() * U * BLOCK + threadIdx.x;
+                                                     M2<float> m, uint32_t chunk) {
+  const uint64_t g0 = xcd_block(chunk) * U * BLOCK + threadIdx.x;
   float4 x[U], y[U];
   uint64_t ii[U], jj[U];
 #pragma unroll
@@ -162,8 +174,8 @@ __global__ __launch_bounds__(BLOCK) void k_1q_f32_q0(float4* __restrict__ a, uin
 // ---- 1q, f64: one double2 per amplitude -----------------------------------
 template <int U, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_1q_f64(double2* __restrict__ a, uint32_t q, uint64_t npairs,
-                                                  M2<double> m) {
-  const uint64_t g0 = flat_block() * U * BLOCK + threadIdx.x;
+                                                  M2<double> m, uint32_t chunk) {
+  const uint64_t g0 = xcd_block(chunk) * U * BLOCK + threadIdx.x;
   double2 x[U], y[U];
   uint64_t ii[U];
 #pragma unroll
@@ -682,6 +694,29 @@ M4<T> to_m4(const rq_c64* m) {
   return r;
 }
 
+// Block chunk of the two-stream 1q kernels by log2 of the distance between the
+// streams in bytes (2^q * amplitude size): the best of 0 (linear order), 64,
+// 128, 256, 512 and 1024 blocks per distance, measured on MI355X with a
+// 30-qubit complex-f32 state (non-temporal access, one group per thread).
+// 1 KB .. 8 KB apart: 5.93-6.16 TB/s linear, 6.36-6.40 with 64.
+constexpr uint16_t XCD_CHUNK_BY_LOG2_DIST[64] = {
+    64,  64,  64,  64,  64,  64,  64,  64,  64,  64,     // 0..9
+    64,  64,  64,  64,  256, 64,  64,  128, 64,  128,    // 10..19: 1 KB 64, 16 KB 256, 32-64 KB 64, 128 KB 128
+    0,   256, 128, 256, 1024, 64, 1024, 256, 128, 64,    // 20..29
+    128, 256, 128, 128, 128, 128, 128, 128, 128, 128,    // 30..39
+    128, 128, 128, 128, 128, 128, 128, 128, 128, 128, 128, 128, 128, 128,
+    128, 128, 128, 128, 128, 128, 128, 128, 128, 128};
+
+// The chunk for a two-stream launch of nb workgroups whose streams are
+// 2^log2d bytes apart: the tuned value or the table, and 0 when nb is not a
+// multiple of 8 chunks.
+uint32_t xcd_chunk_for(uint64_t nb, int log2d) {
+  const int t = stream_tune().xcd_chunk;
+  const uint64_t c = t >= 0 ? (uint64_t)t : XCD_CHUNK_BY_LOG2_DIST[log2d < 63 ? log2d : 63];
+  return (c && nb % (8 * c) == 0) ? (uint32_t)c : 0u;
+}
+uint64_t grid_total(dim3 g) { return (uint64_t)g.x * g.y; }
+
 template <int U, bool NT>
 void launch_1q_t(State& s, uint32_t q, const rq_c64* m) {
   if (s.prec == RQ_C64 && q <= 5 && s.n >= 64) {
@@ -689,8 +724,9 @@ void launch_1q_t(State& s, uint32_t q, const rq_c64* m) {
                        (double2*)s.amps, q, (uint64_t)s.n, to_m2<double>(m));
   } else if (s.prec == RQ_C64) {
     uint64_t np = s.n / 2;
-    hipLaunchKernelGGL((k_1q_f64<U, NT>), grid_for(np, (uint64_t)BLOCK * U), dim3(BLOCK), 0, s.stream,
-                       (double2*)s.amps, q, np, to_m2<double>(m));
+    const dim3 g = grid_for(np, (uint64_t)BLOCK * U);
+    hipLaunchKernelGGL((k_1q_f64<U, NT>), g, dim3(BLOCK), 0, s.stream, (double2*)s.amps, q, np, to_m2<double>(m),
+                       xcd_chunk_for(grid_total(g), (int)q + 4));
   } else if (q == 0) {
     uint64_t np = s.n / 2;
     hipLaunchKernelGGL((k_1q_f32_q0<U, NT>), grid_for(np, (uint64_t)BLOCK * U), dim3(BLOCK), 0, s.stream,
@@ -701,8 +737,9 @@ void launch_1q_t(State& s, uint32_t q, const rq_c64* m) {
                        (float4*)s.amps, q, nv, to_m2<float>(m));
   } else {
     uint64_t ng = s.n / 4;
-    hipLaunchKernelGGL((k_1q_f32_v2<U, NT>), grid_for(ng, (uint64_t)BLOCK * U), dim3(BLOCK), 0, s.stream,
-                       (float4*)s.amps, q, ng, to_m2<float>(m));
+    const dim3 g = grid_for(ng, (uint64_t)BLOCK * U);
+    hipLaunchKernelGGL((k_1q_f32_v2<U, NT>), g, dim3(BLOCK), 0, s.stream, (float4*)s.amps, q, ng, to_m2<float>(m),
+                       xcd_chunk_for(grid_total(g), (int)q + 3));
   }
 }
 

@@ -933,6 +933,11 @@ int rq_tune_set(int key, int value) {
       if (value < 0 || value > 4096) return fail(RQ_ERR_ARGUMENT, "planner seeds out of range");
       fusion_config().seeds = value;
       return RQ_OK;
+    case 6:
+      if (value < -1 || value > 65536 || (value > 0 && (value & (value - 1))))
+        return fail(RQ_ERR_ARGUMENT, "block chunk must be -1, 0 or a power of two up to 65536");
+      stream_tune().xcd_chunk = value;
+      return RQ_OK;
     default:
       return fail(RQ_ERR_ARGUMENT, "unknown tuning key");
   }
@@ -946,6 +951,7 @@ int rq_tune_get(int key) {
     case 3: return fusion_config().c;
     case 4: return fusion_config().mf;
     case 5: return fusion_config().seeds;
+    case 6: return stream_tune().xcd_chunk;
     default: return -2;
   }
 }
