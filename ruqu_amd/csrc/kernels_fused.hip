@@ -260,8 +260,7 @@ __device__ __forceinline__ Cx<float> axpy1(Cx<float> a, float cr, float ci, Cx<f
 // Factored 2x2 on register bit B: [[1, r], [s, 1]] (8 FMAs per pair instead
 // of 16) or, with ANTI, [[r, 1], [1, s]].
 template <int B, class T>
-__device__ __forceinline__ void u1f(const Amps<T>& x, Amps<T>& y, const Cx<T>* __restrict__ mp) {
-  const T rr = mp[0].re, ri = mp[0].im, sr = mp[1].re, si = mp[1].im;
+__device__ __forceinline__ void u1f(const Amps<T>& x, Amps<T>& y, T rr, T ri, T sr, T si) {
 #pragma unroll
   for (int k = 0; k < FUSE_R; ++k) {
     if (k & (1 << B)) continue;
@@ -271,8 +270,7 @@ __device__ __forceinline__ void u1f(const Amps<T>& x, Amps<T>& y, const Cx<T>* _
   }
 }
 template <int B, class T>
-__device__ __forceinline__ void u1f_anti(const Amps<T>& x, Amps<T>& y, const Cx<T>* __restrict__ mp) {
-  const T rr = mp[0].re, ri = mp[0].im, sr = mp[1].re, si = mp[1].im;
+__device__ __forceinline__ void u1f_anti(const Amps<T>& x, Amps<T>& y, T rr, T ri, T sr, T si) {
 #pragma unroll
   for (int k = 0; k < FUSE_R; ++k) {
     if (k & (1 << B)) continue;
@@ -282,14 +280,20 @@ __device__ __forceinline__ void u1f_anti(const Amps<T>& x, Amps<T>& y, const Cx<
   }
 }
 
-// Packed op record (built from FOp at launch, one 8-byte scalar load per op):
+// Packed op record (built from FOp at launch; two records are one 64-byte
+// scalar load):
 //   x: kind:3 | t0:2 | t1:2 | cloc:2 | cpos:5 | dloc0:2 | dpos0:6 | dloc1:2 | dpos1:6
 //   y: matrix / diagonal offset in the pool
+//   c: for a factored 2x2 on a complex-f32 state, its coefficients r and s
+//      (filled in at upload), so the op needs no second, dependent load
 // Locations: 0 none, LOC_REG + 1 register bit, LOC_THR + 1 thread bit,
 // LOC_OUT + 1 tile-id bit (position = bit of blockIdx.x).
 struct POp {
   uint32_t x, y;
+  float c[4];
+  uint32_t pad[2];
 };
+static_assert(sizeof(POp) == 32, "two records per 64-byte scalar load");
 constexpr int PL_NONE = 0, PL_REG = LOC_REG + 1, PL_THR = LOC_THR + 1, PL_OUT = LOC_OUT + 1;
 constexpr int POP_NOP = 6;  // kind of the padding record: copies the amplitudes
 // Factored 2x2 on register bit t0 (pool: r, s): t1 = 0 the unit-diagonal form
@@ -328,22 +332,46 @@ __device__ __forceinline__ uint32_t loc_val(int loc, int pos, uint32_t tid) {
   }
 
 // One op, x -> y.  U1 ops carry no controls (the planner folds only
-// uncontrolled 1-qubit gates into them).
+// uncontrolled 1-qubit gates into them).  Factored 2x2 records, the most
+// frequent kind, are tested first; every other field is decoded in the case
+// that uses it.
 template <class T>
-__device__ __forceinline__ void apply_one(const Amps<T>& x, Amps<T>& y, uint32_t w, const Cx<T>* __restrict__ mp,
+__device__ __forceinline__ void apply_one(const Amps<T>& x, Amps<T>& y, const POp& r, const Cx<T>* __restrict__ pool,
                                           uint32_t tid) {
-  const int kind = w & 7u, t0 = (w >> 3) & 3u, t1 = (w >> 5) & 3u, cl = (w >> 7) & 3u, cp = (w >> 9) & 31u,
-            dl0 = (w >> 14) & 3u, dp0 = (w >> 16) & 63u, dl1 = (w >> 22) & 3u, dp1 = (w >> 24) & 63u;
+  const uint32_t w = r.x;
+  const uint32_t kind = w & 7u;
+  if (__builtin_expect(kind == (uint32_t)POP_U1F, 1)) {  // cloc/cpos: a tile-id bit that toggles the form
+    const int t0 = (w >> 3) & 3u;
+    const uint32_t cl = (w >> 7) & 3u;
+    const uint32_t anti = ((w >> 5) & 1u) ^ (cl == PL_OUT ? loc_val(cl, (w >> 9) & 31u, tid) : 0u);
+    T rr, ri, sr, si;
+    if constexpr (sizeof(T) == 4) {
+      rr = r.c[0]; ri = r.c[1]; sr = r.c[2]; si = r.c[3];
+    } else {
+      const Cx<T>* mp = pool + r.y;
+      rr = mp[0].re; ri = mp[0].im; sr = mp[1].re; si = mp[1].im;
+    }
+    if (anti) {
+      RQ_P1(u1f_anti, t0, x, y, rr, ri, sr, si)
+    } else {
+      RQ_P1(u1f, t0, x, y, rr, ri, sr, si)
+    }
+    return;
+  }
+  const Cx<T>* mp = pool + r.y;
   switch (kind) {
-    case FOP_U1:  // cloc/cpos of a 2x2: a tile-id bit that selects the second of two matrices
+    case FOP_U1: {  // cloc/cpos of a 2x2: a tile-id bit that selects the second of two matrices
+      const int t0 = (w >> 3) & 3u, cl = (w >> 7) & 3u, cp = (w >> 9) & 31u;
       RQ_P1(u1, t0, x, y, (cl == PL_OUT && loc_val(cl, cp, tid)) ? mp + 4 : mp)
       break;
+    }
     case FOP_U2: {
-      const int p0 = t0, p1 = t1;
+      const int p0 = (w >> 3) & 3u, p1 = (w >> 5) & 3u;
       RQ_P2(u2, x, y, mp)
       break;
     }
-    case FOP_CX:
+    case FOP_CX: {
+      const int t0 = (w >> 3) & 3u, cl = (w >> 7) & 3u, cp = (w >> 9) & 31u;
       if (cl == PL_REG) {
         const int p0 = t0, p1 = cp;
         RQ_P2(cx_rc, x, y)
@@ -353,7 +381,9 @@ __device__ __forceinline__ void apply_one(const Amps<T>& x, Amps<T>& y, uint32_t
         RQ_P1(cx_sel, t0, x, y, __builtin_amdgcn_ballot_w64(loc_val(cl, cp, tid) != 0u))
       }
       break;
-    case FOP_CZ:
+    }
+    case FOP_CZ: {
+      const int dl0 = (w >> 14) & 3u, dp0 = (w >> 16) & 63u, dl1 = (w >> 22) & 3u, dp1 = (w >> 24) & 63u;
       if (dl0 == PL_REG && dl1 == PL_REG) {
         const int p0 = dp0, p1 = dp1;
         RQ_P2(cz_rr, x, y)
@@ -365,7 +395,9 @@ __device__ __forceinline__ void apply_one(const Amps<T>& x, Amps<T>& y, uint32_t
         flip_all(x, y, (loc_val(dl0, dp0, tid) & loc_val(dl1, dp1, tid)) << 31);
       }
       break;
-    case FOP_D1:
+    }
+    case FOP_D1: {
+      const int dl0 = (w >> 14) & 3u, dp0 = (w >> 16) & 63u;
       if (dl0 == PL_REG) {
         RQ_P1(dsel_r, dp0, x, y, mp[0], mp[1])
       } else if (dl0 == PL_NONE) {
@@ -374,18 +406,13 @@ __device__ __forceinline__ void apply_one(const Amps<T>& x, Amps<T>& y, uint32_t
         dall(x, y, mp[loc_val(dl0, dp0, tid)]);
       }
       break;
-    case POP_U1F:  // cloc/cpos: a tile-id bit that toggles the form
-      if (t1 ^ (cl == PL_OUT ? loc_val(cl, cp, tid) : 0u)) {
-        RQ_P1(u1f_anti, t0, x, y, mp)
-      } else {
-        RQ_P1(u1f, t0, x, y, mp)
-      }
-      break;
+    }
     case POP_NOP:
 #pragma unroll
       for (int k = 0; k < FUSE_R; ++k) y[k] = fresh(x[k]);
       break;
-    default:  // FOP_D2
+    default: {  // FOP_D2
+      const int dl0 = (w >> 14) & 3u, dp0 = (w >> 16) & 63u, dl1 = (w >> 22) & 3u, dp1 = (w >> 24) & 63u;
       if (dl0 == PL_REG && dl1 == PL_REG) {
         const int p0 = dp0, p1 = dp1;
         RQ_P2(d2_rr, x, y, mp)
@@ -399,6 +426,7 @@ __device__ __forceinline__ void apply_one(const Amps<T>& x, Amps<T>& y, uint32_t
         dall(x, y, mp[2 * loc_val(dl0, dp0, tid) + loc_val(dl1, dp1, tid)]);
       }
       break;
+    }
   }
 }
 
@@ -412,8 +440,8 @@ __device__ __forceinline__ void apply_ops(Amps<T>& v, const POp* __restrict__ op
   Cx<T> w[FUSE_R];
   for (int o = b; o < e; o += 2) {
     const POp p0 = ops[o], p1 = ops[o + 1];
-    apply_one(v, w, p0.x, pool + p0.y, tid);
-    apply_one(w, v, p1.x, pool + p1.y, tid);
+    apply_one(v, w, p0, pool, tid);
+    apply_one(w, v, p1, pool, tid);
   }
 }
 
@@ -1031,7 +1059,7 @@ int pack_ops(const FusedPlan& plan, int mf_min, std::vector<POp>& out, std::vect
       PendingDiag pd;
       auto emit = [&](uint32_t kind, uint32_t t0, uint32_t t1, uint32_t cl, uint32_t cp, uint32_t dl0, uint32_t dp0,
                       uint32_t dl1, uint32_t dp1, uint32_t mat) {
-        POp w;
+        POp w{};
         w.x = kind | (t0 << 3) | (t1 << 5) | (cl << 7) | (cp << 9) | (dl0 << 14) | (dp0 << 16) | (dl1 << 22) |
               (dp1 << 24);
         w.y = mat;
@@ -1043,6 +1071,16 @@ int pack_ops(const FusedPlan& plan, int mf_min, std::vector<POp>& out, std::vect
         for (int k = 0; k < FUSE_R; ++k) pool.push_back(pd.entry(k));
         emit((uint32_t)FOP_D1, 0, 0, 0, 0, (uint32_t)PL_NONE, 0, 0, 0, mat);
         pd = PendingDiag();
+      };
+      // True when an op after o in this phase is a 2x2 or 4x4 on register bit b0 or b1.
+      auto dense_later = [&](int o, int b0, int b1) {
+        for (int q = o + 1; q < plan.phases[ph].op_end; ++q) {
+          const FOp& x = plan.ops[q];
+          if (fold_into[q] >= 0) continue;
+          if (x.kind == FOP_U1 && (x.t0 == b0 || x.t0 == b1)) return true;
+          if (x.kind == FOP_U2 && (x.t0 == b0 || x.t0 == b1 || x.t1 == b0 || x.t1 == b1)) return true;
+        }
+        return false;
       };
       for (int o = plan.phases[ph].op_begin; o < plan.phases[ph].op_end; ++o) {
         const FOp& op = plan.ops[o];
@@ -1145,7 +1183,9 @@ int pack_ops(const FusedPlan& plan, int mf_min, std::vector<POp>& out, std::vect
             emit((uint32_t)FOP_CX, (uint32_t)b, 0, cl, cp, 0, 0, 0, 0, mat);
             break;
           case FOP_CZ:
-            if (dl0 == PL_REG && dl1 == PL_REG) pd.cz((int)dp0, (int)dp1);  // absorbed
+            // Absorbed into the pending diagonal unless a later 2x2 / 4x4 of the phase acts on one of its
+            // bits: the coupling would force a table flush (64 instructions) where the CZ record costs 32.
+            if (dl0 == PL_REG && dl1 == PL_REG && !dense_later(o, (int)dp0, (int)dp1)) pd.cz((int)dp0, (int)dp1);
             else emit((uint32_t)FOP_CZ, 0, 0, 0, 0, dl0, dp0, dl1, dp1, mat);
             break;
           case FOP_D1:
@@ -1159,7 +1199,7 @@ int pack_ops(const FusedPlan& plan, int mf_min, std::vector<POp>& out, std::vect
         }
       }
       flush();
-      if ((out.size() - (size_t)phases[ph].op_begin) & 1) out.push_back(POp{(uint32_t)POP_NOP, 0u});
+      if ((out.size() - (size_t)phases[ph].op_begin) & 1) out.push_back(POp{(uint32_t)POP_NOP, 0u, {0.f, 0.f, 0.f, 0.f}, {0u, 0u}});
       phases[ph].op_end = (int32_t)out.size();
     }
   }
@@ -1208,6 +1248,14 @@ int upload_plan_t(State& s, const FusedPlan& plan, FusedProgram* fp) {
   std::vector<unsigned char> host(total, 0);
   std::memcpy(host.data() + o_st, plan.stages.data(), nst * sizeof(FStage));
   std::memcpy(host.data() + o_ph, phases.data(), nph * sizeof(FPhase));
+  if constexpr (sizeof(T) == 4) {  // factored 2x2 records carry their f32 coefficients inline
+    for (POp& p : pops)
+      if ((p.x & 7u) == (uint32_t)POP_U1F)
+        for (int k = 0; k < 2; ++k) {
+          p.c[2 * k] = (float)poolx[p.y + k].re;
+          p.c[2 * k + 1] = (float)poolx[p.y + k].im;
+        }
+  }
   std::memcpy(host.data() + o_op, pops.data(), nop * sizeof(POp));
   Cx<T>* pp = reinterpret_cast<Cx<T>*>(host.data() + o_pool);
   for (size_t k = 0; k < npool; ++k) pp[k] = Cx<T>{(T)poolx[k].re, (T)poolx[k].im};
