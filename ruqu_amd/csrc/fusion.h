@@ -52,6 +52,8 @@ struct FPhase {
   int32_t thrt[FUSE_MAX_M];      // tile bits holding thread-id bits 0..m-5
   int32_t mf_off;                // -1: ops on the VALU; else pool offset of the phase's matrix-core operands
   uint32_t mf_sel;               // matrix-core phase: the uniform bits that select its matrix (3 x 8 bits)
+  int32_t greg[FUSE_R_BITS];     // global qubits of register bits 0..3 (stage S of regt, set at packing)
+  int32_t gthr[FUSE_MAX_M - FUSE_R_BITS];  // global qubits of thread-id bits 0..m-5 (S of thrt)
 };
 
 struct FStage {

@@ -521,21 +521,26 @@ __device__ __forceinline__ void lds_st(Cx<T>* lds, uint32_t byte, const Cx<T>& x
   *reinterpret_cast<Cx<T>*>(reinterpret_cast<unsigned char*>(lds) + byte) = x;
 }
 
-__device__ __forceinline__ Map make_map(const FStage& st, const int* regt, const int* thrt, uint32_t tid,
-                                        uint64_t outer) {
+__device__ __forceinline__ Map make_map(const FStage& st, const FPhase& ph, uint32_t tid, uint64_t outer) {
+  // Every position comes straight from the phase record (the global ones were
+  // resolved on the host), so the scalar loads issue together instead of as
+  // dependent pairs.
   Map mp;
   mp.tthr = 0;
   mp.gthr = outer;
   const int nt = st.m - FUSE_R_BITS;
-  for (int j = 0; j < nt; ++j) {
-    uint32_t b = (tid >> j) & 1u;
-    mp.tthr |= b << thrt[j];
-    mp.gthr |= (uint64_t)b << st.S[thrt[j]];
+#pragma unroll
+  for (int j = 0; j < FUSE_MAX_M - FUSE_R_BITS; ++j) {
+    if (j < nt) {
+      const uint32_t b = (tid >> j) & 1u;
+      mp.tthr |= b << ph.thrt[j];
+      mp.gthr |= (uint64_t)b << ph.gthr[j];
+    }
   }
 #pragma unroll
   for (int b = 0; b < FUSE_R_BITS; ++b) {
-    mp.treg[b] = 1u << regt[b];
-    mp.greg[b] = 1ull << st.S[regt[b]];
+    mp.treg[b] = 1u << ph.regt[b];
+    mp.greg[b] = 1ull << ph.greg[b];
   }
   return mp;
 }
@@ -608,7 +613,9 @@ __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FS
   uint64_t outer = 0;
   {
     const uint64_t tile = blockIdx.x;
-    for (int j = 0; j < st.nouter; ++j) outer |= ((tile >> j) & 1ull) << st.outer[j];
+#pragma unroll
+    for (int j = 0; j < 32; ++j)  // tiles have at most 31 id bits; the loads issue together
+      if (j < st.nouter) outer |= ((tile >> j) & 1ull) << st.outer[j];
   }
   // Wave priority: high while moving data (HBM loads and stores, LDS
   // transposes), low while applying ops.  The tiles of a stage all do the
@@ -620,7 +627,7 @@ __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FS
   const int np = st.nphase;
   for (int p = 0; p < np; ++p) {
     const FPhase& ph = phases[st.phase0 + p];
-    const Map mp = make_map(st, ph.regt, ph.thrt, tid, outer);
+    const Map mp = make_map(st, ph, tid, outer);
     if (p == 0) {
       if (ph.direct) {
         Unroll<FUSE_R, T>::gather_g(v, a, mp);
@@ -1203,6 +1210,12 @@ int pack_ops(const FusedPlan& plan, int mf_min, std::vector<POp>& out, std::vect
       phases[ph].op_end = (int32_t)out.size();
     }
   }
+  for (const FStage& st : plan.stages)  // global qubits of each phase's register and thread bits
+    for (int ph = st.phase0; ph < st.phase0 + st.nphase; ++ph) {
+      FPhase& f = phases[ph];
+      for (int b = 0; b < FUSE_R_BITS; ++b) f.greg[b] = st.S[f.regt[b]];
+      for (int j = 0; j < st.m - FUSE_R_BITS; ++j) f.gthr[j] = st.S[f.thrt[j]];
+    }
   return RQ_OK;
 }
 
