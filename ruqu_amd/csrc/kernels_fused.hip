@@ -343,7 +343,8 @@ __device__ __forceinline__ void apply_one(const Amps<T>& x, Amps<T>& y, const PO
   if (__builtin_expect(kind == (uint32_t)POP_U1F, 1)) {  // cloc/cpos: a tile-id bit that toggles the form
     const int t0 = (w >> 3) & 3u;
     const uint32_t cl = (w >> 7) & 3u;
-    const uint32_t anti = ((w >> 5) & 1u) ^ (cl == PL_OUT ? loc_val(cl, (w >> 9) & 31u, tid) : 0u);
+    // the toggle is a tile-id bit (never a thread bit): wave-uniform, a scalar branch
+    const uint32_t anti = ((w >> 5) & 1u) ^ (cl == PL_OUT ? (blockIdx.x >> ((w >> 9) & 31u)) & 1u : 0u);
     T rr, ri, sr, si;
     if constexpr (sizeof(T) == 4) {
       rr = r.c[0]; ri = r.c[1]; sr = r.c[2]; si = r.c[3];
