@@ -420,8 +420,11 @@ int rq_fusion_packed_kinds(uint32_t num_qubits, int precision, const rq_gate* ga
  * key 3 = fused low qubits c, key 4 = fused matrix-core threshold: complex-f32
  * phases whose ops cost at least this many VALU instructions per wave run as
  * one composed 16x16 unitary on the matrix cores (-1: never), key 5 = tile-qubit
- * seeds the fused planner tries per stage (0..4096; 0: plain greedy scan; the
- * plan stays valid for any value, only the stage count changes).
+ * seeds the fused planner tries per stage (0..4096, default 512; 0: plain
+ * greedy scan; the plan stays valid for any value, only the stage count
+ * changes), key 6 = workgroup order of the two-stream exact 1q kernels (-1:
+ * the measured per-distance chunk table, 0: linear, a power of two: each XCD
+ * takes that many consecutive blocks at a time).
  * rq_tune_get returns -2 for an unknown key. */
 int rq_tune_set(int key, int value);
 int rq_tune_get(int key);

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <thread>
 
 #include "fusion.h"
 #include "gates.h"
@@ -187,13 +188,32 @@ int plan_fused(uint32_t nq, const rq_gate* gates, size_t ngates, const FusionCon
       }
       constexpr size_t SEED_WINDOW = 384;
       int best = scan(low, nullptr, nullptr, SEED_WINDOW).first;
+      // The seeds in RNG order, then their scores on host threads (each scan
+      // only reads the items), then the first best in seed order: the plan
+      // does not depend on the thread count.
+      std::vector<uint64_t> cand;
       for (int r = 0; r < cfg.seeds && nfront > m - popc(low); ++r) {
         for (int k = nfront - 1; k > 0; --k) std::swap(front[k], front[next_rand() % (uint64_t)(k + 1)]);
         uint64_t S0 = low;
         for (int k = 0; k < nfront && popc(S0) < m; ++k) S0 |= 1ull << front[k];
-        const int got = scan(S0, nullptr, nullptr, SEED_WINDOW).first;
-        if (got > best) best = got, seed = S0;
+        cand.push_back(S0);
       }
+      std::vector<int> score(cand.size());
+      const unsigned hw = std::thread::hardware_concurrency();
+      const size_t nt = std::min<size_t>({hw ? hw : 1u, 16u, (cand.size() + 31) / 32});
+      auto work = [&](size_t t) {
+        for (size_t i = t; i < cand.size(); i += nt) score[i] = scan(cand[i], nullptr, nullptr, SEED_WINDOW).first;
+      };
+      if (nt <= 1) {
+        work(0);
+      } else {
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nt; ++t) pool.emplace_back(work, t);
+        work(0);
+        for (auto& th : pool) th.join();
+      }
+      for (size_t i = 0; i < cand.size(); ++i)
+        if (score[i] > best) best = score[i], seed = cand[i];
     }
     std::vector<int> ops, deferred;
     uint64_t S = scan(seed, &ops, &deferred, remaining.size()).second;

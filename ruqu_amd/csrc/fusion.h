@@ -78,7 +78,7 @@ struct FusionConfig {
   int c = 5;       // lowest qubits always in the tile
   int mf = 768;    // complex-f32 phases whose ops cost at least this many VALU instructions per wave run on
                    // the matrix cores as one composed 16x16 unitary; -1: never
-  int seeds = 128; // random tile-qubit seeds tried per stage (0: the plain greedy scan only)
+  int seeds = 512; // random tile-qubit seeds tried per stage (0: the plain greedy scan only)
 };
 
 // Executes gates[0..ngates) with the fused schedule; Measure/Reset gates run
