@@ -306,6 +306,47 @@ def save_records(path, rec):
     np.savez_compressed(path, n=ns, ne=ne, edges=edges, w=w, p=rec["p"], gammas=G, betas=B, best=rec["best"])
 
 
+def save_records_compact(path, rec, start):
+    """Records of the consecutive training instances start .. start+count-1
+    without their graphs: the graphs are a function of the index
+    (instances()), so only p, the angles and the best cut are stored."""
+    G = np.zeros((len(rec["p"]), PMAX))
+    B = np.zeros((len(rec["p"]), PMAX))
+    for i, p in enumerate(rec["p"]):
+        G[i, :p] = rec["gammas"][i]
+        B[i, :p] = rec["betas"][i]
+    np.savez_compressed(path, start=np.int64(start), p=np.asarray(rec["p"], dtype=np.int8), gammas=G, betas=B,
+                        best=np.asarray(rec["best"]))
+
+
+def load_records_compact(paths):
+    """Concatenate compact record files, regenerating each graph from its
+    instance index; the regenerated depth must equal the stored one."""
+    out = {"graphs": [], "p": [], "gammas": [], "betas": [], "best": []}
+    for path in paths:
+        with np.load(path) as z:
+            arrs = {k: z[k] for k in z.files}
+        p = arrs["p"].astype(np.int64)
+        insts = instances(len(p), start=int(arrs["start"]))
+        for i, (n, e, w, pi) in enumerate(insts):
+            if pi != p[i]:
+                raise ValueError("%s: record %d has p=%d, instance %d has p=%d" % (path, i, p[i], i, pi))
+            out["graphs"].append((n, e, w))
+            out["gammas"].append(arrs["gammas"][i, :pi])
+            out["betas"].append(arrs["betas"][i, :pi])
+        out["p"].append(p)
+        out["best"].append(arrs["best"])
+    out["p"] = np.concatenate(out["p"])
+    out["best"] = np.concatenate(out["best"])
+    return out
+
+
+def subset(rec, count):
+    """The first count records."""
+    return {"graphs": rec["graphs"][:count], "p": rec["p"][:count], "gammas": rec["gammas"][:count],
+            "betas": rec["betas"][:count], "best": rec["best"][:count]}
+
+
 def load_records(path):
     with np.load(path) as z:  # each z[...] access decompresses: read every array once
         arrs = {k: z[k] for k in z.files}
@@ -326,7 +367,14 @@ def main(argv=None):
     import os
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--records", type=int, default=49152)
+    ap.add_argument("--start", type=int, default=0, help="first training instance index of new records")
+    ap.add_argument("--records-only", action="store_true",
+                    help="make records (saved compact, without graphs) and stop")
     ap.add_argument("--record-file", default=None, help="reuse records saved by an earlier run")
+    ap.add_argument("--compact-records", nargs="*", default=None,
+                    help="reuse compact record files (graphs regenerated from the instance index)")
+    ap.add_argument("--curve", default=None,
+                    help="comma-separated record counts: train one network per prefix and evaluate each")
     ap.add_argument("--params", default=None, help="start from (or, with --epochs 0, evaluate) these parameters")
     ap.add_argument("--iters", type=int, default=16, help="plain optimizer iterations")
     ap.add_argument("--epochs", type=int, default=30)
@@ -342,12 +390,21 @@ def main(argv=None):
 
     t0 = time.time()
     rec = None
-    if a.record_file:
+    if a.compact_records:
+        rec = load_records_compact(a.compact_records)
+        log("loaded %d compact records (%.0f s)" % (len(rec["p"]), time.time() - t0))
+    elif a.record_file:
         rec = load_records(a.record_file)
         log("loaded %d records" % len(rec["p"]))
     elif a.records > 0:
-        rec = make_records(a.records, a.iters, log=log)
+        rec = make_records(a.records, a.iters, start=a.start, log=log)
+        if a.records_only:
+            save_records_compact(os.path.join(a.out, "records_%d_%d.npz" % (a.start, a.records)), rec, a.start)
+            log("records done in %.0f s" % (time.time() - t0))
+            return
         save_records(os.path.join(a.out, "records.npz"), rec)
+    if a.curve:
+        return learning_curve(rec, [int(c) for c in a.curve.split(",")], a, log)
     t1 = time.time()
     net = Network.load(a.params) if a.params else Network(seed=a.seed)
     hist = train(net, rec, a.epochs, lr=a.lr, seed=a.seed, log=log) if (a.epochs and rec) else []
@@ -368,6 +425,40 @@ def main(argv=None):
     with open(os.path.join(a.out, "result.json"), "w") as f:
         json.dump(res, f, indent=1)
     log("done in %.0f s" % (time.time() - t0))
+
+
+def learning_curve(rec, counts, a, log):
+    """One network per record prefix (same epochs, learning rate and seed),
+    each judged alone and with the per-depth constant of its own prefix on
+    the same held-out instances against one plain baseline."""
+    import json
+    import os
+    ev = held_out(a.eval)
+    base = plain_baseline(ev, a.iters)
+    log("plain baseline: %d instances, %.1f s, mean ratio %.4f" % (len(ev), base["wall_s"], float(np.mean(base["ratio"]))))
+    rows = []
+    for c in counts:
+        sub = subset(rec, c)
+        net = Network(seed=a.seed)
+        t0 = time.time()
+        hist = train(net, sub, a.epochs, lr=a.lr, seed=a.seed, log=None)
+        t_train = time.time() - t0
+        net.save(os.path.join(a.out, "params_%d.npy" % c))
+        const = constant_predictor(sub)
+        alone = evaluate(net, ev, a.iters, log=None, plain=base)
+        pair = evaluate([net, const], ev, a.iters, log=None, plain=base, rescue_iters=6)
+        row = {"records": c, "epochs": a.epochs, "train_s": t_train, "final_loss": hist[-1],
+               "net_within_1pct": alone["within_1pct_energy"], "net_ratio_raw": alone["mean_ratio_warm_raw"],
+               "net_speedup": alone["speedup"], "net_const_within_1pct": pair["within_1pct_energy"],
+               "net_const_speedup": pair["speedup"], "net_const_fallbacks": pair["fallbacks"]}
+        rows.append(row)
+        log("curve %6d records: loss %.5f, train %.0f s | network %.1f%% within 1%%, raw ratio %.4f, %.2fx | "
+            "network+constant (rescue 6) %.1f%%, %.2fx, %d fallbacks" % (
+                c, row["final_loss"], t_train, 100 * row["net_within_1pct"], row["net_ratio_raw"], row["net_speedup"],
+                100 * row["net_const_within_1pct"], row["net_const_speedup"], row["net_const_fallbacks"]))
+        with open(os.path.join(a.out, "curve.json"), "w") as f:
+            json.dump(rows, f, indent=1)
+    return rows
 
 
 if __name__ == "__main__":
