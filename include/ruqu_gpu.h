@@ -413,6 +413,13 @@ int rq_fusion_packed_info(uint32_t num_qubits, int precision, const rq_gate* gat
  * tables over the register bits. */
 int rq_fusion_packed_kinds(uint32_t num_qubits, int precision, const rq_gate* gates, size_t ngates, int m, int c,
                            uint64_t* counts);
+/* Host-only: that program's records: the first 32-bit word of each (kind:3 |
+ * t0:2 | t1:2 | cloc:2 | cpos:5 | dloc0:2 | dpos0:6 | dloc1:2 | dpos1:6;
+ * locations 0 none, 1 register bit, 2 thread bit, 3 tile-id bit) and the
+ * phase each belongs to (-1: a matrix-core phase's record).  count receives
+ * the number of records; RQ_ERR_ARGUMENT when it exceeds cap. */
+int rq_fusion_packed_records(uint32_t num_qubits, int precision, const rq_gate* gates, size_t ngates, int m, int c,
+                             uint32_t* words, int32_t* phase, size_t cap, size_t* count);
 
 /* Process-wide tuning knobs (no reference counterpart; set by the autotuner):
  * key 0 = streaming kernels' amplitude groups per thread (1, 2, 4, 8),

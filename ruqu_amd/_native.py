@@ -108,6 +108,7 @@ SIGNATURES = {
     "rq_fusion_emulate": (_i, [_u32, _i, _vp, _sz, _i, _i, _vp]),
     "rq_fusion_packed_info": (_i, [_u32, _i, _vp, _sz, _i, _i, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)]),
     "rq_fusion_packed_kinds": (_i, [_u32, _i, _vp, _sz, _i, _i, _vp]),
+    "rq_fusion_packed_records": (_i, [_u32, _i, _vp, _sz, _i, _i, _vp, _vp, _sz, _vp]),
     "rq_tune_set": (_i, [_i, _i]),
     "rq_tune_get": (_i, [_i]),
     "rq_time_1q": (_i, [_vp, _u32, _vp, _i, _pd]),

@@ -457,6 +457,20 @@ int rq_fusion_packed_kinds(uint32_t nq, int precision, const rq_gate* gates, siz
   return rq::packed_kinds(plan, precision, rq::fusion_config().mf, counts);
 }
 
+// The packed records themselves: word x of each and the phase it belongs to
+// (-1 for the records of matrix-core phases).  Host only.
+int rq_fusion_packed_records(uint32_t nq, int precision, const rq_gate* gates, size_t ngates, int m, int c,
+                             uint32_t* words, int32_t* phase, size_t cap, size_t* count) {
+  if (!words || !phase || !count) return rq::fail(RQ_ERR_ARGUMENT, "null pointer");
+  rq::FusedPlan plan;
+  rq::FusionConfig cfg = rq::fusion_config();
+  cfg.m = m;
+  cfg.c = c;
+  int rc = rq::plan_fused(nq, gates, ngates, cfg, &plan);
+  if (rc) return rc;
+  return rq::packed_records(plan, precision, rq::fusion_config().mf, words, phase, cap, count);
+}
+
 // Sets the tile size (m qubits) and the count of low qubits always kept in a
 // tile (c) used by the fused schedule of rq_run_circuit.
 int rq_fusion_configure(int m, int c) {

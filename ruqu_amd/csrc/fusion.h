@@ -116,6 +116,8 @@ int emulate_packed(const FusedPlan& plan, int precision, int mf_min, void* amps)
 // 4x4, 1-bit diagonal, 2-bit diagonal, CNOT, CZ, padding, factored 2x2),
 // counts[8] the pending-diagonal tables (counted apart from kind 2).
 int packed_kinds(const FusedPlan& plan, int precision, int mf_min, uint64_t* counts);
+int packed_records(const FusedPlan& plan, int precision, int mf_min, uint32_t* words, int32_t* phase, size_t cap,
+                   size_t* count);
 int packed_info(const FusedPlan& plan, int precision, int mf_min, uint64_t* records, uint64_t* mf_phases,
                 uint64_t* mf_unitaries);
 

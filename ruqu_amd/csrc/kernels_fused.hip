@@ -1504,6 +1504,24 @@ int packed_kinds(const FusedPlan& plan, int precision, int mf_min, uint64_t* cou
   return RQ_OK;
 }
 
+int packed_records(const FusedPlan& plan, int precision, int mf_min, uint32_t* words, int32_t* phase, size_t cap,
+                   size_t* count) {
+  std::vector<POp> pops;
+  std::vector<FPhase> phases;
+  std::vector<rq_c64> poolx;
+  if (int rc = pack_ops(plan, precision == RQ_C64 ? -1 : mf_min, pops, phases, poolx)) return rc;
+  *count = pops.size();
+  if (pops.size() > cap) return fail(RQ_ERR_ARGUMENT, "record buffer too small");
+  for (size_t i = 0; i < pops.size(); ++i) {
+    words[i] = pops[i].x;
+    phase[i] = -1;
+  }
+  for (size_t k = 0; k < phases.size(); ++k)
+    if (phases[k].mf_off < 0)
+      for (int32_t o = phases[k].op_begin; o < phases[k].op_end; ++o) phase[o] = (int32_t)k;
+  return RQ_OK;
+}
+
 int launch_fused_plan(State& s, const FusedPlan& plan) {
   if (plan.stages.empty()) return RQ_OK;
   if (int rc = check_shapes(plan)) return rc;

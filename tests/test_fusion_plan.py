@@ -137,3 +137,32 @@ def test_factored_records_replace_dense_2x2():
     assert max_abs(emulate(n, arr, 10, 4), want) < 1e-12
     got32 = emulate(n, arr, 10, 4, "c32")
     assert max_abs(got32, want) < 1e-5
+
+
+def test_packed_records_export():
+    """rq_fusion_packed_records lists the same program rq_fusion_packed_kinds
+    counts: the kinds agree, every VALU phase holds an even number of records
+    (odd ones are padded), and a too-small buffer is refused."""
+    n = 14
+    arr = workloads.random_circuit(n, 10, 21).to_array()
+    cnt = np.zeros(9, dtype=np.uint64)
+    assert lib().rq_fusion_packed_kinds(n, 1, ptr(arr), len(arr), 10, 4, ptr(cnt)) == 0
+    cap = 1 << 14
+    words = np.zeros(cap, dtype=np.uint32)
+    phase = np.zeros(cap, dtype=np.int32)
+    count = C.c_size_t()
+    assert lib().rq_fusion_packed_records(n, 1, ptr(arr), len(arr), 10, 4, ptr(words), ptr(phase), cap,
+                                          C.byref(count)) == 0
+    k = count.value
+    assert k == int(cnt.sum())
+    words, phase = words[:k], phase[:k]
+    kind = words & 7
+    table = (kind == 2) & (((words >> 14) & 3) == 0)
+    got = np.bincount(np.where(table, 8, kind), minlength=9)
+    assert np.array_equal(got, cnt.astype(np.int64))
+    _, per_phase = np.unique(phase[phase >= 0], return_counts=True)
+    assert np.all(per_phase % 2 == 0)
+    small = C.c_size_t()
+    assert lib().rq_fusion_packed_records(n, 1, ptr(arr), len(arr), 10, 4, ptr(words), ptr(phase), k - 1,
+                                          C.byref(small)) != 0
+    assert small.value == k
