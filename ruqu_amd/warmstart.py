@@ -274,23 +274,26 @@ class ConstantPredictor:
         return [self.table[p] for p in ps]
 
 
+def constant_table(records):
+    """Mean recorded angles per depth, in the format of
+    ruqu_amd/data/warmstart_constant.json (0.5 for a depth with no record)."""
+    table = {}
+    ps = np.asarray(records["p"])
+    for p in range(1, PMAX + 1):
+        sel = np.flatnonzero(ps == p)
+        if len(sel):
+            g = np.mean([records["gammas"][i] for i in sel], axis=0)
+            b = np.mean([records["betas"][i] for i in sel], axis=0)
+        else:
+            g, b = np.full(p, 0.5), np.full(p, 0.5)
+        table[str(p)] = {"gammas": [float(x) for x in g], "betas": [float(x) for x in b], "records": int(len(sel))}
+    return table
+
+
 def constant_predictor(records):
     """The method without the network: the mean recorded angles per depth
     (same for every graph), as an ablation of what the graph input adds."""
-    means = {}
-    for p in range(1, PMAX + 1):
-        sel = [i for i in range(len(records["p"])) if records["p"][i] == p]
-        if sel:
-            means[p] = (np.mean([records["gammas"][i] for i in sel], axis=0),
-                        np.mean([records["betas"][i] for i in sel], axis=0))
-        else:
-            means[p] = (np.full(p, 0.5), np.full(p, 0.5))
-
-    class _Const:
-        def predict(self, graphs, ps):
-            return [means[p] for p in ps]
-
-    return _Const()
+    return ConstantPredictor(constant_table(records))
 
 
 def save_records(path, rec):
@@ -444,7 +447,10 @@ def learning_curve(rec, counts, a, log):
         hist = train(net, sub, a.epochs, lr=a.lr, seed=a.seed, log=None)
         t_train = time.time() - t0
         net.save(os.path.join(a.out, "params_%d.npy" % c))
-        const = constant_predictor(sub)
+        table = constant_table(sub)
+        with open(os.path.join(a.out, "constant_%d.json" % c), "w") as f:
+            json.dump(table, f, indent=1)
+        const = ConstantPredictor(table)
         alone = evaluate(net, ev, a.iters, log=None, plain=base)
         pair = evaluate([net, const], ev, a.iters, log=None, plain=base, rescue_iters=6)
         row = {"records": c, "epochs": a.epochs, "train_s": t_train, "final_loss": hist[-1],
