@@ -45,16 +45,20 @@ struct FOp {
   int32_t dpos0, dpos1;  // bit index within that location
 };
 
+// One phase of a stage.  128 bytes, the fields the kernel reads first, so
+// that a workgroup fetches a phase with two 64-byte scalar loads.
 struct FPhase {
   int32_t op_begin, op_end;
   int32_t direct;                // 1: as first / last phase, access HBM in this phase's mapping
-  int32_t regt[FUSE_R_BITS];     // tile bits holding register bits 0..3
-  int32_t thrt[FUSE_MAX_M];      // tile bits holding thread-id bits 0..m-5
   int32_t mf_off;                // -1: ops on the VALU; else pool offset of the phase's matrix-core operands
   uint32_t mf_sel;               // matrix-core phase: the uniform bits that select its matrix (3 x 8 bits)
+  int32_t regt[FUSE_R_BITS];     // tile bits holding register bits 0..3
   int32_t greg[FUSE_R_BITS];     // global qubits of register bits 0..3 (stage S of regt, set at packing)
+  int32_t thrt[FUSE_MAX_M - FUSE_R_BITS];  // tile bits holding thread-id bits 0..m-5 (0 past m-5)
   int32_t gthr[FUSE_MAX_M - FUSE_R_BITS];  // global qubits of thread-id bits 0..m-5 (S of thrt)
+  int32_t pad[3];
 };
+static_assert(sizeof(FPhase) == 128, "phase record: two 64-byte scalar loads");
 
 struct FStage {
   int32_t m;            // tile qubits

@@ -14,6 +14,7 @@
 //
 // Arithmetic: FP contraction is allowed here (fused mode), so results differ
 // from the per-gate kernels at the rounding level.
+#include <cstddef>
 #include <cstring>
 #include <utility>
 
@@ -116,16 +117,20 @@ __device__ __forceinline__ uint32_t fresh32(uint32_t a) {
   asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(a));
   return r;
 }
+__device__ __forceinline__ uint64_t fresh64(uint64_t a) {
+  uint64_t r;
+  asm("v_mov_b64 %0, %1" : "=v"(r) : "v"(a));
+  return r;
+}
 template <class T>
 __device__ __forceinline__ Cx<T> fresh(Cx<T> a) {
   if constexpr (sizeof(T) == 4) {
-    return Cx<T>{__uint_as_float(fresh32(__float_as_uint(a.re))), __uint_as_float(fresh32(__float_as_uint(a.im)))};
+    const uint64_t u = ((uint64_t)__float_as_uint(a.im) << 32) | __float_as_uint(a.re);
+    const uint64_t r = fresh64(u);
+    return Cx<T>{__uint_as_float((uint32_t)r), __uint_as_float((uint32_t)(r >> 32))};
   } else {
-    auto f = [](double d) {
-      const unsigned long long u = __double_as_longlong(d);
-      return __longlong_as_double(((unsigned long long)fresh32((uint32_t)(u >> 32)) << 32) | fresh32((uint32_t)u));
-    };
-    return Cx<T>{f(a.re), f(a.im)};
+    return Cx<T>{__longlong_as_double(fresh64(__double_as_longlong(a.re))),
+                 __longlong_as_double(fresh64(__double_as_longlong(a.im)))};
   }
 }
 
@@ -258,26 +263,47 @@ __device__ __forceinline__ Cx<float> axpy1(Cx<float> a, float cr, float ci, Cx<f
 }
 
 // Factored 2x2 on register bit B: [[1, r], [s, 1]] (8 FMAs per pair instead
-// of 16) or, with ANTI, [[r, 1], [1, s]].
-template <int B, class T>
-__device__ __forceinline__ void u1f(const Amps<T>& x, Amps<T>& y, T rr, T ri, T sr, T si) {
+// of 16) or, with ANTI, [[r, 1], [1, s]].  Every output is a + c b in two
+// FMAs; all first FMAs are issued before any second one.  Each input then
+// stays live until a second FMA reads it as a multiplicand, so no first FMA
+// can overwrite its addend's register (two-address form): the outputs are
+// written straight into the other register set, with no copies.
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+template <class T>
+__device__ __forceinline__ void axpy_first(Cx<T>& t, Cx<T> a, T cr, Cx<T> b) {
+  t.re = fma_t(cr, b.re, a.re);
+  t.im = fma_t(cr, b.im, a.im);
+}
+template <class T>
+__device__ __forceinline__ void axpy_second(Cx<T>& t, T ci, Cx<T> b) {
+  t.re = fma_t(-ci, b.im, t.re);
+  t.im = fma_t(ci, b.re, t.im);
+}
+template <int B, bool ANTI, class T>
+__device__ __forceinline__ void u1f_t(const Amps<T>& x, Amps<T>& y, T rr, T ri, T sr, T si) {
 #pragma unroll
   for (int k = 0; k < FUSE_R; ++k) {
     if (k & (1 << B)) continue;
     const int k1 = k | (1 << B);
-    y[k] = axpy1(x[k], rr, ri, x[k1]);
-    y[k1] = axpy1(x[k1], sr, si, x[k]);
+    axpy_first(y[k], ANTI ? x[k1] : x[k], rr, ANTI ? x[k] : x[k1]);
+    axpy_first(y[k1], ANTI ? x[k] : x[k1], sr, ANTI ? x[k1] : x[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) {
+    if (k & (1 << B)) continue;
+    const int k1 = k | (1 << B);
+    axpy_second(y[k], ri, ANTI ? x[k] : x[k1]);
+    axpy_second(y[k1], si, ANTI ? x[k1] : x[k]);
   }
 }
 template <int B, class T>
+__device__ __forceinline__ void u1f(const Amps<T>& x, Amps<T>& y, T rr, T ri, T sr, T si) {
+  u1f_t<B, false>(x, y, rr, ri, sr, si);
+}
+template <int B, class T>
 __device__ __forceinline__ void u1f_anti(const Amps<T>& x, Amps<T>& y, T rr, T ri, T sr, T si) {
-#pragma unroll
-  for (int k = 0; k < FUSE_R; ++k) {
-    if (k & (1 << B)) continue;
-    const int k1 = k | (1 << B);
-    y[k] = axpy1(x[k1], rr, ri, x[k]);
-    y[k1] = axpy1(x[k], sr, si, x[k1]);
-  }
+  u1f_t<B, true>(x, y, rr, ri, sr, si);
 }
 
 // Packed op record (built from FOp at launch; two records are one 64-byte
@@ -342,9 +368,10 @@ __device__ __forceinline__ void apply_one(const Amps<T>& x, Amps<T>& y, const PO
   const uint32_t kind = w & 7u;
   if (__builtin_expect(kind == (uint32_t)POP_U1F, 1)) {  // cloc/cpos: a tile-id bit that toggles the form
     const int t0 = (w >> 3) & 3u;
-    const uint32_t cl = (w >> 7) & 3u;
-    // the toggle is a tile-id bit (never a thread bit): wave-uniform, a scalar branch
-    const uint32_t anti = ((w >> 5) & 1u) ^ (cl == PL_OUT ? (blockIdx.x >> ((w >> 9) & 31u)) & 1u : 0u);
+    // the toggle is a tile-id bit (never a thread bit): wave-uniform, a
+    // scalar branch.  Integer arithmetic only (cloc == PL_OUT: bits 7 and 8
+    // both set), so the test stays on the scalar unit.
+    const uint32_t anti = ((w >> 5) ^ ((blockIdx.x >> ((w >> 9) & 31u)) & (w >> 7) & (w >> 8))) & 1u;
     T rr, ri, sr, si;
     if constexpr (sizeof(T) == 4) {
       rr = r.c[0]; ri = r.c[1]; sr = r.c[2]; si = r.c[3];
@@ -440,7 +467,26 @@ __device__ __forceinline__ void apply_ops(Amps<T>& v, const POp* __restrict__ op
                                           const Cx<T>* __restrict__ pool, uint32_t tid) {
   Cx<T> w[FUSE_R];
   for (int o = b; o < e; o += 2) {
-    const POp p0 = ops[o], p1 = ops[o + 1];
+    POp p0, p1;
+    if constexpr (sizeof(T) == 4) {
+      // both records in one 64-byte scalar load at the top of the iteration:
+      // the inline coefficients of a factored 2x2 are then in registers when
+      // its branch is taken, instead of a second load after the kind test
+      typedef uint32_t u32x16 __attribute__((ext_vector_type(16)));
+      const u32x16 raw = *reinterpret_cast<const u32x16*>(ops + o);
+      p0.x = raw[0];
+      p0.y = raw[1];
+      p1.x = raw[8];
+      p1.y = raw[9];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        p0.c[i] = __uint_as_float(raw[2 + i]);
+        p1.c[i] = __uint_as_float(raw[10 + i]);
+      }
+    } else {
+      p0 = ops[o];
+      p1 = ops[o + 1];
+    }
     apply_one(v, w, p0, pool, tid);
     apply_one(w, v, p1, pool, tid);
   }
@@ -522,26 +568,43 @@ __device__ __forceinline__ void lds_st(Cx<T>* lds, uint32_t byte, const Cx<T>& x
   *reinterpret_cast<Cx<T>*>(reinterpret_cast<unsigned char*>(lds) + byte) = x;
 }
 
-__device__ __forceinline__ Map make_map(const FStage& st, const FPhase& ph, uint32_t tid, uint64_t outer) {
-  // Every position comes straight from the phase record (the global ones were
-  // resolved on the host), so the scalar loads issue together instead of as
-  // dependent pairs.
+// A phase record as two 64-byte scalar loads (dword d of the record is
+// word(d)); every field the kernel reads is then in registers at once.
+typedef int32_t i32x16 __attribute__((ext_vector_type(16)));
+struct PhaseWords {
+  i32x16 lo, hi;
+  __device__ __forceinline__ int32_t word(int d) const { return d < 16 ? lo[d] : hi[d - 16]; }
+  __device__ __forceinline__ int32_t op_begin() const { return word(offsetof(FPhase, op_begin) / 4); }
+  __device__ __forceinline__ int32_t op_end() const { return word(offsetof(FPhase, op_end) / 4); }
+  __device__ __forceinline__ int32_t direct() const { return word(offsetof(FPhase, direct) / 4); }
+  __device__ __forceinline__ int32_t mf_off() const { return word(offsetof(FPhase, mf_off) / 4); }
+  __device__ __forceinline__ uint32_t mf_sel() const { return (uint32_t)word(offsetof(FPhase, mf_sel) / 4); }
+  __device__ __forceinline__ int32_t regt(int b) const { return word(offsetof(FPhase, regt) / 4 + b); }
+  __device__ __forceinline__ int32_t greg(int b) const { return word(offsetof(FPhase, greg) / 4 + b); }
+  __device__ __forceinline__ int32_t thrt(int j) const { return word(offsetof(FPhase, thrt) / 4 + j); }
+  __device__ __forceinline__ int32_t gthr(int j) const { return word(offsetof(FPhase, gthr) / 4 + j); }
+};
+__device__ __forceinline__ PhaseWords load_phase(const FPhase* p) {
+  const i32x16* v = reinterpret_cast<const i32x16*>(p);
+  return PhaseWords{v[0], v[1]};
+}
+
+// Branch-free: the block has 2^(m-4) threads, so thread-id bits past m-5 are
+// zero, and the record's unused position entries are zero.
+__device__ __forceinline__ Map make_map(const PhaseWords& ph, uint32_t tid, uint64_t outer) {
   Map mp;
   mp.tthr = 0;
   mp.gthr = outer;
-  const int nt = st.m - FUSE_R_BITS;
 #pragma unroll
   for (int j = 0; j < FUSE_MAX_M - FUSE_R_BITS; ++j) {
-    if (j < nt) {
-      const uint32_t b = (tid >> j) & 1u;
-      mp.tthr |= b << ph.thrt[j];
-      mp.gthr |= (uint64_t)b << ph.gthr[j];
-    }
+    const uint32_t b = (tid >> j) & 1u;
+    mp.tthr |= b << (ph.thrt(j) & 31);
+    mp.gthr |= (uint64_t)b << (ph.gthr(j) & 63);
   }
 #pragma unroll
   for (int b = 0; b < FUSE_R_BITS; ++b) {
-    mp.treg[b] = 1u << ph.regt[b];
-    mp.greg[b] = 1ull << ph.greg[b];
+    mp.treg[b] = 1u << ph.regt(b);
+    mp.greg[b] = 1ull << ph.greg(b);
   }
   return mp;
 }
@@ -603,7 +666,7 @@ struct Unroll<0, T> {
 };
 
 template <class T>
-__global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FStage* __restrict__ stp,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_fused_stage(T* __restrict__ a, const FStage* __restrict__ stp,
                                                       const FPhase* __restrict__ phases, const POp* __restrict__ ops,
                                                       const Cx<T>* __restrict__ pool) {
   // A static array (the kernel's only LDS) sits at LDS address 0, so the
@@ -613,10 +676,13 @@ __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FS
   const uint32_t tid = threadIdx.x;
   uint64_t outer = 0;
   {
+    // tiles have at most 31 id bits: outer[0..31] in two 64-byte scalar
+    // loads; bits of blockIdx.x past nouter are zero, so no test is needed
+    const i32x16* ov = reinterpret_cast<const i32x16*>(st.outer);
+    const i32x16 o0 = ov[0], o1 = ov[1];
     const uint64_t tile = blockIdx.x;
 #pragma unroll
-    for (int j = 0; j < 32; ++j)  // tiles have at most 31 id bits; the loads issue together
-      if (j < st.nouter) outer |= ((tile >> j) & 1ull) << st.outer[j];
+    for (int j = 0; j < 32; ++j) outer |= ((tile >> j) & 1ull) << ((j < 16 ? o0[j] : o1[j - 16]) & 63);
   }
   // Wave priority: high while moving data (HBM loads and stores, LDS
   // transposes), low while applying ops.  The tiles of a stage all do the
@@ -627,10 +693,10 @@ __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FS
   Cx<T> v[FUSE_R];
   const int np = st.nphase;
   for (int p = 0; p < np; ++p) {
-    const FPhase& ph = phases[st.phase0 + p];
-    const Map mp = make_map(st, ph, tid, outer);
+    const PhaseWords ph = load_phase(phases + st.phase0 + p);
+    const Map mp = make_map(ph, tid, outer);
     if (p == 0) {
-      if (ph.direct) {
+      if (ph.direct()) {
         Unroll<FUSE_R, T>::gather_g(v, a, mp);
       } else {
         const Map nat = make_natural_map(st, tid, outer);
@@ -646,14 +712,14 @@ __global__ __launch_bounds__(256) void k_fused_stage(T* __restrict__ a, const FS
     }
     __builtin_amdgcn_s_setprio(0);
     if constexpr (sizeof(T) == 4) {
-      if (ph.mf_off >= 0) apply_mf(v, reinterpret_cast<const float*>(pool + ph.mf_off), ph.mf_sel, tid);
-      else apply_ops(v, ops, ph.op_begin, ph.op_end, pool, tid);
+      if (ph.mf_off() >= 0) apply_mf(v, reinterpret_cast<const float*>(pool + ph.mf_off()), ph.mf_sel(), tid);
+      else apply_ops(v, ops, ph.op_begin(), ph.op_end(), pool, tid);
     } else {
-      apply_ops(v, ops, ph.op_begin, ph.op_end, pool, tid);
+      apply_ops(v, ops, ph.op_begin(), ph.op_end(), pool, tid);
     }
     __builtin_amdgcn_s_setprio(3);
     if (p == np - 1) {
-      if (ph.direct) {
+      if (ph.direct()) {
         Unroll<FUSE_R, T>::scatter_g(v, a, mp);
       } else {
         const Map nat = make_natural_map(st, tid, outer);
