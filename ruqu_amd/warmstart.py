@@ -32,8 +32,9 @@ GRAPH_STREAM = 1 << 30
 
 
 def default_params_path():
-    """The shipped network: trained on 49 152 plain-optimizer records
-    (16 iterations each) of this module's training stream."""
+    """The shipped network: trained on the 200 000 plain-optimizer records
+    (16 iterations each) of training instances 0..199 999 of this module's
+    stream, 30 epochs, Adam lr 1e-3, 256 graphs per step."""
     import os
     return os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "warmstart_params.npy")
 
@@ -369,7 +370,7 @@ def main(argv=None):
     import json
     import os
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--records", type=int, default=49152)
+    ap.add_argument("--records", type=int, default=200000)
     ap.add_argument("--start", type=int, default=0, help="first training instance index of new records")
     ap.add_argument("--records-only", action="store_true",
                     help="make records (saved compact, without graphs) and stop")
