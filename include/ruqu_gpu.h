@@ -363,18 +363,22 @@ int rq_batch_maxcut(rq_batch* b, double* out);
 /* HIP-event time of the last evaluate/optimize call. */
 double rq_batch_last_ms(const rq_batch* b);
 /* LDS tile of the states above 13 qubits: 2^tq amplitudes per workgroup,
- * ceil(n_max/2) <= tq <= 13 (no reference counterpart; the autotuner's knob). */
+ * ceil(n_max/2) <= tq <= 13 (no reference counterpart; the autotuner's knob).
+ * A state of n qubits whose second tiling would read runs of fewer than 16
+ * amplitudes (2 tq - n < 4) gets a wider tile, up to 13 qubits. */
 int rq_batch_set_tile(rq_batch* b, int tq);
+/* The same with every state above 13 qubits on tiles of exactly tq qubits. */
+int rq_batch_set_tile_exact(rq_batch* b, int tq);
 int rq_batch_get_tile(const rq_batch* b);
 /* Host-only inspection (no device call; no reference counterpart): the launch
  * plan of one round (round < 0: the plain evaluation; else round r of an
  * optimizer iteration) as the device receives it.  info[8] = {nsmall, nwide,
- * ntiles, npasses, nrounds, nouts, tq, nvtiles}; info64[2] = {amplitudes of
+ * ntiles, nlaunches, nrounds, nouts, tq, nvtiles}; info64[2] = {amplitudes of
  * the round's wide states, cut-table floats}; small/wide items are 16-byte
  * records; wtile0/wstate: first tile and state offset per wide item; wmap:
- * wide item of each tile; pass_grid: tiles launched at each pass; vmap: the
- * round's tiles of VQE items.  Null outputs are skipped (call once for the
- * counts). */
+ * wide item of each tile; pass_grid: tiles of each layer-pass launch (one per
+ * pass and tile width); vmap: the round's tiles of VQE items.  Null outputs
+ * are skipped (call once for the counts). */
 int rq_batch_plan_export(rq_batch* b, int round, int32_t* info, int64_t* info64, void* small_items,
                          void* wide_items, int32_t* wtile0, int64_t* wstate, int32_t* wmap, int32_t* pass_grid,
                          int32_t* vmap);
@@ -383,10 +387,13 @@ int rq_batch_plan_export(rq_batch* b, int round, int32_t* info, int64_t* info64,
 int rq_batch_data_export(rq_batch* b, void* insts, void* edges, void* terms, double* params, size_t* counts);
 /* Host-only: the pass structure of a round's wide items (states above 13
  * qubits): first / last pass, the state the first pass reads (a shared prefix
- * state when the first pass is > 0), and every pass's tile list
- * (concatenated; pass s at pass_off[s]). */
+ * state when the first pass is > 0), and every launch's tile list
+ * (concatenated; launch i at pass_off[i]). */
 int rq_batch_plan_passes(rq_batch* b, int round, int32_t* wfirst, int32_t* wlast, int64_t* wsrc, int32_t* ptile,
                          int32_t* pass_off);
+/* Host-only: the pass and the tile qubits of each layer-pass launch of a
+ * round (nlaunches entries each). */
+int rq_batch_plan_launches(rq_batch* b, int round, int32_t* pass, int32_t* tile_qubits);
 
 /* ---- fused schedule (no reference counterpart: the reference's optimizer
  * only folds adjacent 1q gates, optimizer.rs:46-94, and is not called by

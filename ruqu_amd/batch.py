@@ -31,6 +31,7 @@ _native.register("rq_batch_set_all_params", C.c_int, [C.c_void_p, C.c_void_p, C.
 _native.register("rq_batch_size", C.c_uint32, [C.c_void_p])
 _native.register("rq_batch_last_ms", C.c_double, [C.c_void_p])
 _native.register("rq_batch_set_tile", C.c_int, [C.c_void_p, C.c_int])
+_native.register("rq_batch_set_tile_exact", C.c_int, [C.c_void_p, C.c_int])
 _native.register("rq_batch_maxcut", C.c_int, [C.c_void_p, C.c_void_p])
 _native.register("rq_batch_results", C.c_int, [C.c_void_p] + [C.c_void_p] * 6)
 _native.register("rq_batch_get_tile", C.c_int, [C.c_void_p])
@@ -155,9 +156,12 @@ class Batch:
                 "params": [prm[offs[i]:offs[i + 1]] for i in range(n)],
                 "best_params": [bprm[offs[i]:offs[i + 1]] for i in range(n)]}
 
-    def set_tile(self, tq):
-        """LDS tile (2^tq amplitudes per workgroup) of the states above 13 qubits."""
-        check(lib().rq_batch_set_tile(self._h, int(tq)))
+    def set_tile(self, tq, exact=False):
+        """LDS tile (2^tq amplitudes per workgroup) of the states above 13
+        qubits; unless exact, a state whose second tiling would read runs
+        shorter than a 128-byte line gets a wider tile (up to 13 qubits)."""
+        f = lib().rq_batch_set_tile_exact if exact else lib().rq_batch_set_tile
+        check(f(self._h, int(tq)))
 
     def tile(self):
         return int(lib().rq_batch_get_tile(self._h))

@@ -42,9 +42,9 @@ __global__ void k_wide_pass(const BInst* insts, const BItem* witems, const int32
                             const float* ctab, float2* states, int pass, BPart* parts);
 __global__ void k_wide_energy(const BInst* insts, const BItem* witems, const int32_t* wmap, const int32_t* wtile0,
                               const int64_t* wstate, const int32_t* vmap, const int32_t* done, const BTerm* terms,
-                              const float2* states, int tq, BPart* parts);
+                              const float2* states, BPart* parts);
 __global__ void k_wide_fold(const BInst* insts, const BItem* witems, const int32_t* wtile0, const int32_t* done,
-                            const BPart* parts, int tq, double* energy, uint32_t* amax);
+                            const BPart* parts, double* energy, uint32_t* amax);
 __global__ void k_cut_tables(float* ctab, const BInst* insts, const BEdge* edges, const BCutChunk* chunks,
                              uint32_t nchunks);
 __global__ void k_maxcut(int n, const BEdge* e, int ne, unsigned long long* out);
@@ -150,8 +150,10 @@ struct RoundPlan {
   int small_off = 0, nsmall = 0;  // into all_small
   int wide_off = 0, nwide = 0;    // into all_wide / all_wtile0 / all_wstate / all_wfirst / all_wlast / all_wsrc
   int tile_off = 0, ntiles = 0;   // into all_wmap
-  std::vector<int> layer_grid;    // pass s: number of tiles of the items whose passes include s
-  std::vector<int> pass_off;      // pass s: its tile list in all_ptile
+  // k_wide_pass launches, pass by pass and, within a pass, one per tile width:
+  // the pass, the tile qubits, the number of tiles and the tile list's offset
+  // in all_ptile
+  std::vector<int> launch_pass, launch_tq, layer_grid, pass_off;
   int vmap_off = 0, nvtiles = 0;  // into all_vmap: the round's VQE tiles
   int64_t state_amps = 0;         // HBM amplitudes needed (prefix states included)
 };
@@ -174,6 +176,7 @@ struct Batch {
   bool planned = false;   // host plan built for the current tile width
   bool plan_on_device = false;
   int tq = BATCH_TILE_QUBITS;
+  bool tile_exact = false;  // every wide state on tiles of tq qubits (else batch_item_tq)
   int nwg = 0;
   hipStream_t st = nullptr;
   BInst* d_insts = nullptr;
@@ -271,7 +274,7 @@ RoundPlan plan_round(Batch& b, const std::vector<BItem>& v, std::vector<ItemPass
   for (int j = 0; j < rp.nwide; ++j) {
     const BInst& in = b.insts[wide[j].first.inst];
     const ItemPass& ip = wide[j].second;
-    const int nt = 1 << (in.n - b.tq);
+    const int nt = 1 << (in.n - in.tq);
     b.all_wide.push_back(wide[j].first);
     b.all_wtile0.push_back(tiles);
     b.all_wfirst.push_back(ip.first);
@@ -290,19 +293,24 @@ RoundPlan plan_round(Batch& b, const std::vector<BItem>& v, std::vector<ItemPass
     tiles += nt;
     maxpass = std::max(maxpass, ip.last);
   }
-  // pass s: the tiles of every item whose passes include s, item by item
-  rp.layer_grid.assign(maxpass + 1, 0);
-  rp.pass_off.assign(maxpass + 1, 0);
-  for (int s = 0; s <= maxpass; ++s) {
-    rp.pass_off[s] = (int)b.all_ptile.size();
-    for (int j = 0; j < rp.nwide; ++j) {
-      const ItemPass& ip = wide[j].second;
-      if (s < ip.first || s > ip.last) continue;
-      const int t0 = b.all_wtile0[rp.wide_off + j], nt = 1 << (b.insts[wide[j].first.inst].n - b.tq);
-      for (int t = 0; t < nt; ++t) b.all_ptile.push_back(t0 + t);
+  // pass s, tile width w: the tiles of every item on w-qubit tiles whose
+  // passes include s, item by item; one launch each (widest first)
+  for (int s = 0; s <= maxpass; ++s)
+    for (int w = BATCH_MAX_TILE_QUBITS; w >= BATCH_MIN_TILE_QUBITS; --w) {
+      const int off = (int)b.all_ptile.size();
+      for (int j = 0; j < rp.nwide; ++j) {
+        const ItemPass& ip = wide[j].second;
+        const BInst& in = b.insts[wide[j].first.inst];
+        if (in.tq != w || s < ip.first || s > ip.last) continue;
+        const int t0 = b.all_wtile0[rp.wide_off + j], nt = 1 << (in.n - in.tq);
+        for (int t = 0; t < nt; ++t) b.all_ptile.push_back(t0 + t);
+      }
+      if ((int)b.all_ptile.size() == off) continue;
+      rp.launch_pass.push_back(s);
+      rp.launch_tq.push_back(w);
+      rp.layer_grid.push_back((int)b.all_ptile.size() - off);
+      rp.pass_off.push_back(off);
     }
-    rp.layer_grid[s] = (int)b.all_ptile.size() - rp.pass_off[s];
-  }
   rp.nvtiles = (int)b.all_vmap.size() - rp.vmap_off;
   rp.ntiles = tiles;
   rp.state_amps = amps;
@@ -317,6 +325,8 @@ void build_plan(Batch& b) {
   for (const BInst& in : b.insts)
     if (in.n > BATCH_LDS_QUBITS) nmax = std::max(nmax, in.n);
   b.tq = std::max(b.tq, (nmax + 1) / 2);  // two tile passes per layer
+  for (BInst& in : b.insts)
+    in.tq = in.n > BATCH_LDS_QUBITS ? (b.tile_exact ? b.tq : batch_item_tq(in.n, b.tq)) : 0;
   b.ctab_total = 0;
   for (BInst& in : b.insts) {
     if (in.kind == BK_QAOA) {  // C(i) tables for every QAOA state, LDS-resident ones included
@@ -453,6 +463,8 @@ void free_dev(T*& p) {
 
 // Uploads the launch plan and sizes the wide-state and partial buffers.
 int upload_plan(Batch& b) {
+  if (b.d_insts)  // the instances' tile widths change with the plan
+    RQ_HIP(hipMemcpy(b.d_insts, b.insts.data(), b.insts.size() * sizeof(BInst), hipMemcpyHostToDevice));
   free_dev(b.d_small);
   free_dev(b.d_wide);
   free_dev(b.d_wtile0);
@@ -571,13 +583,13 @@ void enqueue_round(Batch& b, const RoundPlan& rp, unsigned* counter, const int32
   const int32_t* wm = b.d_wmap + rp.tile_off;
   const int32_t* w0 = b.d_wtile0 + rp.wide_off;
   const int64_t* ws = b.d_wstate + rp.wide_off;
-  const size_t lds = wide_lds_bytes(b.tq);
-  for (int ps = 0; ps < (int)rp.layer_grid.size(); ++ps) {
-    if (!rp.layer_grid[ps]) continue;
-    const dim3 g(rp.layer_grid[ps]), t(BATCH_THREADS);
-    const WidePasses wp{b.d_ptile + rp.pass_off[ps], b.d_wfirst + rp.wide_off, b.d_wlast + rp.wide_off,
+  for (size_t li = 0; li < rp.layer_grid.size(); ++li) {
+    const int ps = rp.launch_pass[li], tq = rp.launch_tq[li];
+    const size_t lds = wide_lds_bytes(tq);
+    const dim3 g(rp.layer_grid[li]), t(BATCH_THREADS);
+    const WidePasses wp{b.d_ptile + rp.pass_off[li], b.d_wfirst + rp.wide_off, b.d_wlast + rp.wide_off,
                         b.d_wsrc + rp.wide_off};
-    switch (b.tq) {
+    switch (tq) {
       case 9: hipLaunchKernelGGL(k_wide_pass<9>, g, t, lds, b.st, b.d_insts, wi, wm, w0, ws, wp, done, b.d_params,
                                  b.d_ctab, b.d_states, ps, b.d_parts); break;
       case 10: hipLaunchKernelGGL(k_wide_pass<10>, g, t, lds, b.st, b.d_insts, wi, wm, w0, ws, wp, done, b.d_params,
@@ -592,9 +604,9 @@ void enqueue_round(Batch& b, const RoundPlan& rp, unsigned* counter, const int32
   }
   if (rp.nvtiles)
     hipLaunchKernelGGL(k_wide_energy, dim3(rp.nvtiles), dim3(BATCH_THREADS), 0, b.st, b.d_insts, wi, wm, w0, ws,
-                       b.d_vmap + rp.vmap_off, done, b.d_terms, b.d_states, b.tq, b.d_parts);
+                       b.d_vmap + rp.vmap_off, done, b.d_terms, b.d_states, b.d_parts);
   hipLaunchKernelGGL(k_wide_fold, dim3(rp.nwide), dim3(BATCH_THREADS), 0, b.st, b.d_insts, wi, w0, done, b.d_parts,
-                     b.tq, b.d_energy, b.d_amax);
+                     b.d_energy, b.d_amax);
 }
 
 void enqueue_iteration(Batch& b, int max_iter, double qaoa_lr, double vqe_lr, double vqe_thr) {
@@ -852,10 +864,9 @@ int rq_batch_set_all_params(rq_batch* h, const double* params, size_t count) {
   return RQ_OK;
 }
 
-// LDS tile width of the wide (n > 13) states: 2^tq amplitudes per
-// workgroup.  Needs ceil(n_max / 2) <= tq <= 13 (two passes per layer).
-int rq_batch_set_tile(rq_batch* h, int tq) {
-  Batch* b = (Batch*)h;
+namespace {
+
+int set_tile(Batch* b, int tq, bool exact) {
   if (!b) return fail(RQ_ERR_ARGUMENT, "null batch");
   int nmax = 0;
   for (const BInst& in : b->insts)
@@ -863,9 +874,10 @@ int rq_batch_set_tile(rq_batch* h, int tq) {
   if (tq < BATCH_MIN_TILE_QUBITS || tq > BATCH_MAX_TILE_QUBITS || 2 * tq < nmax)
     return fail(RQ_ERR_ARGUMENT, "tile qubits must be in [max(" + std::to_string(BATCH_MIN_TILE_QUBITS) +
                                      ", ceil(n_max/2)), " + std::to_string(BATCH_MAX_TILE_QUBITS) + "]");
-  if (tq == b->tq) return RQ_OK;
+  if (tq == b->tq && exact == b->tile_exact) return RQ_OK;
   if (b->st) RQ_HIP(hipStreamSynchronize(b->st));
   b->tq = tq;
+  b->tile_exact = exact;
   b->planned = false;
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
@@ -874,6 +886,17 @@ int rq_batch_set_tile(rq_batch* h, int tq) {
   return RQ_OK;
 }
 
+}  // namespace
+
+// LDS tile width of the wide (n > 13) states: 2^tq amplitudes per
+// workgroup, widened per state (batch_item_tq) up to 13 qubits where the
+// second tiling would otherwise read runs shorter than a 128-byte line.
+// Needs ceil(n_max / 2) <= tq <= 13 (two passes per layer).
+int rq_batch_set_tile(rq_batch* h, int tq) { return set_tile((Batch*)h, tq, false); }
+
+// The same with every wide state on tiles of exactly tq qubits.
+int rq_batch_set_tile_exact(rq_batch* h, int tq) { return set_tile((Batch*)h, tq, true); }
+
 int rq_batch_get_tile(const rq_batch* h) {
   const Batch* b = (const Batch*)h;
   return b ? b->tq : -1;
@@ -881,9 +904,11 @@ int rq_batch_get_tile(const rq_batch* h) {
 
 // Host-only (no device call): the launch plan of one round (round < 0: the
 // plain evaluation, else round r of an optimizer iteration), as the device
-// receives it.  info[0..7] = nsmall, nwide, ntiles, npasses, nrounds, nouts,
-// tq, nvtiles; info64[0..1] = HBM amplitudes of the round's wide states,
-// cut-table floats.  Null arrays are skipped (call once for the counts).
+// receives it.  info[0..7] = nsmall, nwide, ntiles, nlaunches (k_wide_pass
+// launches: one per pass and tile width), nrounds, nouts, tq (the batch's
+// base tile), nvtiles; info64[0..1] = HBM amplitudes of the round's wide
+// states, cut-table floats; pass_grid = tiles per launch.  Null arrays are
+// skipped (call once for the counts).
 int rq_batch_plan_export(rq_batch* h, int round, int32_t* info, int64_t* info64, void* small_items,
                          void* wide_items, int32_t* wtile0, int64_t* wstate, int32_t* wmap, int32_t* pass_grid,
                          int32_t* vmap) {
@@ -911,9 +936,22 @@ int rq_batch_plan_export(rq_batch* h, int round, int32_t* info, int64_t* info64,
   return RQ_OK;
 }
 
+// Host-only: the k_wide_pass launches of one round: the pass and the tile
+// qubits of each.
+int rq_batch_plan_launches(rq_batch* h, int round, int32_t* pass, int32_t* tile_qubits) {
+  Batch* b = (Batch*)h;
+  if (!b || b->insts.empty()) return fail(RQ_ERR_ARGUMENT, "empty batch");
+  if (!b->planned) build_plan(*b);
+  if (round >= b->nrounds) return fail(RQ_ERR_ARGUMENT, "round out of range");
+  const RoundPlan& rp = round < 0 ? b->base : b->rounds[round];
+  if (pass) std::copy(rp.launch_pass.begin(), rp.launch_pass.end(), pass);
+  if (tile_qubits) std::copy(rp.launch_tq.begin(), rp.launch_tq.end(), tile_qubits);
+  return RQ_OK;
+}
+
 // Host-only: the pass structure of one round's wide items: first / last pass
-// and first-pass source state per item, and the tile list of every pass
-// (concatenated; pass s starts at pass_off[s]).
+// and first-pass source state per item, and the tile list of every launch
+// (concatenated; launch i starts at pass_off[i]).
 int rq_batch_plan_passes(rq_batch* h, int round, int32_t* wfirst, int32_t* wlast, int64_t* wsrc, int32_t* ptile,
                          int32_t* pass_off) {
   Batch* b = (Batch*)h;

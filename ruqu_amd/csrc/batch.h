@@ -18,6 +18,15 @@ constexpr int BATCH_CUT_CHUNK = 4096;      // cut-table entries per workgroup of
 
 enum BatchKind : int32_t { BK_QAOA = 0, BK_VQE = 1 };
 
+// Tile qubits of a wide state of n qubits when the batch's tile is `base`:
+// the tiling B pass reads runs of 2^(2 tq - n) amplitudes, so the tile widens
+// (up to BATCH_MAX_TILE_QUBITS) until a run fills a 128-byte line (16
+// complex-f32 amplitudes, 2 tq - n >= 4).
+inline int batch_item_tq(int n, int base) {
+  const int need = (n + 5) / 2;
+  return need > base ? (need < BATCH_MAX_TILE_QUBITS ? need : BATCH_MAX_TILE_QUBITS) : base;
+}
+
 struct BInst {
   int32_t kind;
   int32_t n;
@@ -26,7 +35,7 @@ struct BInst {
   int32_t nterms;     // QAOA edges or VQE Pauli terms
   int32_t term_off;   // first edge / term
   int32_t param_off;  // first parameter (doubles)
-  int32_t pad;
+  int32_t tq;         // n > BATCH_LDS_QUBITS: tile qubits of the state's passes (set when planning), else 0
   int64_t ctab_off;   // QAOA, n > BATCH_LDS_QUBITS: offset of the cut table (floats), else -1
   double wsum;        // QAOA: total edge weight
 };

@@ -7,7 +7,8 @@
 //            takes the next work item from a device counter (items sorted
 //            largest first, so uneven sizes share one launch without tails).
 //   n >  13 ("wide"): every state of the round has its own region in HBM and
-//            is spread over many workgroups, one per tile of 2^tq amplitudes;
+//            is spread over many workgroups, one per tile of 2^tq amplitudes
+//            (tq per state: BInst.tq);
 //            k_wide_pass makes p + 1 passes for p layers (see below), the
 //            last QAOA pass yields per-tile FP64 energy partials, VQE energies
 //            come from k_wide_energy, and k_wide_fold folds each item's
@@ -629,7 +630,7 @@ __global__ __launch_bounds__(B) void k_wide_energy(const BInst* __restrict__ ins
                                                    const int64_t* __restrict__ wstate,
                                                    const int32_t* __restrict__ vmap,
                                                    const int32_t* __restrict__ done, const BTerm* __restrict__ terms,
-                                                   const float2* __restrict__ states, int tq,
+                                                   const float2* __restrict__ states,
                                                    BPart* __restrict__ parts) {
   __shared__ Shared sh;
   const int tile = vmap[blockIdx.x];
@@ -637,7 +638,7 @@ __global__ __launch_bounds__(B) void k_wide_energy(const BInst* __restrict__ ins
   const BItem it = witems[j];
   const BInst in = insts[it.inst];
   if (done[it.inst]) return;
-  const uint32_t T = 1u << tq, base = (uint32_t)(tile - wtile0[j]) << tq;
+  const uint32_t T = 1u << in.tq, base = (uint32_t)(tile - wtile0[j]) << in.tq;
   const float2* __restrict__ g = states + wstate[j];
   const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
   const BTerm* __restrict__ tt = terms + in.term_off;
@@ -660,7 +661,7 @@ __global__ __launch_bounds__(B) void k_wide_energy(const BInst* __restrict__ ins
 // on scheduling) into energy[out] and amax[out].
 __global__ __launch_bounds__(B) void k_wide_fold(const BInst* __restrict__ insts, const BItem* __restrict__ witems,
                                                  const int32_t* __restrict__ wtile0, const int32_t* __restrict__ done,
-                                                 const BPart* __restrict__ parts, int tq, double* __restrict__ energy,
+                                                 const BPart* __restrict__ parts, double* __restrict__ energy,
                                                  uint32_t* __restrict__ amax) {
   __shared__ Shared sh;
   const int j = blockIdx.x;
@@ -675,7 +676,7 @@ __global__ __launch_bounds__(B) void k_wide_fold(const BInst* __restrict__ insts
     }
     return;
   }
-  const int ntiles = 1 << (in.n - tq), t0 = wtile0[j];
+  const int ntiles = 1 << (in.n - in.tq), t0 = wtile0[j];
   double acc = 0.0, bv = -1.0;
   uint32_t bi = 0;
   for (int t = threadIdx.x; t < ntiles; t += B) {

@@ -120,14 +120,14 @@ int emu_wide_pass(int grid, const rq::BInst* insts, const rq::BItem* wi, const i
 
 int emu_wide_energy(int grid, const rq::BInst* insts, const rq::BItem* wi, const int32_t* wmap,
                     const int32_t* wtile0, const int64_t* wstate, const int32_t* vmap, const int32_t* done,
-                    const rq::BTerm* terms, const float2* states, int tq, rq::BPart* parts) {
-  run_grid(grid, [&] { rq::k_wide_energy(insts, wi, wmap, wtile0, wstate, vmap, done, terms, states, tq, parts); });
+                    const rq::BTerm* terms, const float2* states, rq::BPart* parts) {
+  run_grid(grid, [&] { rq::k_wide_energy(insts, wi, wmap, wtile0, wstate, vmap, done, terms, states, parts); });
   return 0;
 }
 
 int emu_wide_fold(int grid, const rq::BInst* insts, const rq::BItem* wi, const int32_t* wtile0, const int32_t* done,
-                  const rq::BPart* parts, int tq, double* energy, uint32_t* amax) {
-  run_grid(grid, [&] { rq::k_wide_fold(insts, wi, wtile0, done, parts, tq, energy, amax); });
+                  const rq::BPart* parts, double* energy, uint32_t* amax) {
+  run_grid(grid, [&] { rq::k_wide_fold(insts, wi, wtile0, done, parts, energy, amax); });
   return 0;
 }
 

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libbatch_emu.so")
 
 INST = np.dtype([("kind", "<i4"), ("n", "<i4"), ("p", "<i4"), ("nparams", "<i4"), ("nterms", "<i4"),
-                 ("term_off", "<i4"), ("param_off", "<i4"), ("pad", "<i4"), ("ctab_off", "<i8"), ("wsum", "<f8")])
+                 ("term_off", "<i4"), ("param_off", "<i4"), ("tq", "<i4"), ("ctab_off", "<i8"), ("wsum", "<f8")])
 ITEM = np.dtype([("inst", "<i4"), ("variant", "<i4"), ("param", "<i4"), ("out", "<i4")])
 EDGE = np.dtype([("i", "<u4"), ("j", "<u4"), ("w", "<f8")])
 TERM = np.dtype([("x", "<u8"), ("zy", "<u8"), ("ny", "<u4"), ("pad", "<u4"), ("c", "<f8")])
@@ -37,8 +37,8 @@ def _emu():
     lib.emu_cut_tables.argtypes = [vp, vp, vp, vp, i]
     lib.emu_batch_eval.argtypes = [vp, vp, i] + [vp] * 7 + [i]
     lib.emu_wide_pass.argtypes = [i] + [vp] * 9 + [i, i, vp] + [vp] * 4
-    lib.emu_wide_energy.argtypes = [i] + [vp] * 9 + [i, vp]
-    lib.emu_wide_fold.argtypes = [i] + [vp] * 5 + [i, vp, vp]
+    lib.emu_wide_energy.argtypes = [i] + [vp] * 9 + [vp]
+    lib.emu_wide_fold.argtypes = [i] + [vp] * 5 + [vp, vp]
     return lib
 
 
@@ -83,6 +83,9 @@ def plan(batch, rnd=-1):
              pass_off=np.zeros(max(npasses, 1), dtype=np.int32))
     assert lib.rq_batch_plan_passes(batch._h, rnd, ip(r["wfirst"]), ip(r["wlast"]), lp(r["wsrc"]), ip(r["ptile"]),
                                     ip(r["pass_off"])) == 0
+    lib.rq_batch_plan_launches.argtypes = [C.c_void_p, C.c_int, I32P, I32P]
+    r.update(launch_pass=np.zeros(max(npasses, 1), dtype=np.int32), launch_tq=np.zeros(max(npasses, 1), dtype=np.int32))
+    assert lib.rq_batch_plan_launches(batch._h, rnd, ip(r["launch_pass"]), ip(r["launch_tq"])) == 0
     return r
 
 
@@ -107,16 +110,15 @@ def _run_round(lib, d, r, ctab, states, energy, amax, grid):
     if r["nwide"]:
         parts = np.zeros(r["ntiles"], dtype=PART)
         w = (_p(insts), _p(r["wide"]), _p(r["wmap"]), _p(r["wtile0"]), _p(r["wstate"]), _p(done))
-        for ps, g in enumerate(r["pass_grid"]):
+        for li, g in enumerate(r["pass_grid"]):  # one launch per pass and tile width
             if g:
-                lib.emu_wide_pass(int(g), *w, _p(pa), _p(ctab), _p(states), r["tq"], ps, _p(parts),
-                                  _p(r["ptile"][int(r["pass_off"][ps]):]), _p(r["wfirst"]), _p(r["wlast"]),
-                                  _p(r["wsrc"]))
+                lib.emu_wide_pass(int(g), *w, _p(pa), _p(ctab), _p(states), int(r["launch_tq"][li]),
+                                  int(r["launch_pass"][li]), _p(parts), _p(r["ptile"][int(r["pass_off"][li]):]),
+                                  _p(r["wfirst"]), _p(r["wlast"]), _p(r["wsrc"]))
         if r["nvtiles"]:
-            lib.emu_wide_energy(r["nvtiles"], *w[:5], _p(r["vmap"]), _p(done), _p(ta), _p(states), r["tq"],
-                                _p(parts))
-        lib.emu_wide_fold(r["nwide"], _p(insts), _p(r["wide"]), _p(r["wtile0"]), _p(done), _p(parts), r["tq"],
-                          _p(energy), _p(amax))
+            lib.emu_wide_energy(r["nvtiles"], *w[:5], _p(r["vmap"]), _p(done), _p(ta), _p(states), _p(parts))
+        lib.emu_wide_fold(r["nwide"], _p(insts), _p(r["wide"]), _p(r["wtile0"]), _p(done), _p(parts), _p(energy),
+                          _p(amax))
 
 
 def evaluate(batch, grid=3):

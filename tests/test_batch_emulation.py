@@ -105,11 +105,42 @@ def test_batch_wide_edge_tilings_on_cpu():
         rng = np.random.default_rng(70 + k)
         qc.append((n, e, w, p, list(rng.uniform(0, math.pi, p)), list(rng.uniform(0, math.pi, p))))
         b.add_qaoa(*qc[-1])
-    b.set_tile(9)
+    b.set_tile(9, exact=True)
     en, am = batch_emu.evaluate(b)
     for k, (n, e, w, p, g, be) in enumerate(qc):
         want, amps = orc.qaoa_cost(n, e, w, g, be, want_state=True)
         assert abs(en[k] - want) <= 1e-5 * (1 + sum(w)), (n, en[k], want)
+        pr = np.abs(amps) ** 2
+        assert pr[am[k]] >= pr.max() * (1 - 1e-4)
+
+
+def test_batch_widens_tiles_that_would_cut_lines():
+    """With base tile 10, a 20-qubit state's second tiling would read runs of
+    one amplitude: it gets 12-qubit tiles (runs of 16 amplitudes, whole
+    128-byte lines) while a 15-qubit state keeps 10; every pass then launches
+    once per tile width.  Energies equal the oracle's, as with uniform tiles."""
+    b = Batch()
+    qc = []
+    for k, (n, p) in enumerate([(15, 2), (20, 2)]):
+        e, w = workloads.erdos_renyi_graph(n, 0.4, 600 + k)
+        rng = np.random.default_rng(90 + k)
+        qc.append((n, e, w, p, list(rng.uniform(0, math.pi, p)), list(rng.uniform(0, math.pi, p))))
+        b.add_qaoa(*qc[-1])
+    b.set_tile(10)
+    assert list(batch_emu.data(b)["insts"]["tq"]) == [10, 12]
+    r = batch_emu.plan(b, -1)
+    assert list(r["launch_pass"]) == [0, 0, 1, 1, 2, 2] and list(r["launch_tq"]) == [12, 10] * 3
+    assert list(r["pass_grid"]) == [1 << 8, 1 << 5] * 3
+    assert r["ntiles"] == (1 << 8) + (1 << 5)
+    en, am = batch_emu.evaluate(b)
+    b.set_tile(10, exact=True)
+    assert list(batch_emu.data(b)["insts"]["tq"]) == [10, 10]
+    assert list(batch_emu.plan(b, -1)["launch_tq"]) == [10] * 3
+    en_exact, _ = batch_emu.evaluate(b)
+    for k, (n, e, w, p, g, be) in enumerate(qc):
+        want, amps = orc.qaoa_cost(n, e, w, g, be, want_state=True)
+        tol = 1e-5 * (1 + sum(w))
+        assert abs(en[k] - want) <= tol and abs(en_exact[k] - want) <= tol, (n, en[k], en_exact[k], want)
         pr = np.abs(amps) ** 2
         assert pr[am[k]] >= pr.max() * (1 - 1e-4)
 

@@ -115,10 +115,13 @@ def test_batch_optimizer_follows_reference_loops(use_graph):
     assert abs(got["best"] - best) < 1e-4
 
 
+@pytest.mark.parametrize("exact", [True, False])
 @pytest.mark.parametrize("tq", [9, 11, 12, 13])
-def test_batch_tile_widths_match_oracle(tq):
-    """Wide states at every tile width, including tiling B with 0-3 low bits
-    (n close to 2*tq: tiles sharing 128-byte lines, the XCD-swizzled order)."""
+def test_batch_tile_widths_match_oracle(tq, exact):
+    """Wide states at every tile width.  Exact tiles include tiling B with
+    0-3 low bits (n close to 2*tq: tiles sharing 128-byte lines, the
+    XCD-swizzled order); widened tiles put the larger states on wider tiles,
+    so every pass launches once per tile width."""
     sizes = [(14, 3), (16, 1), (17, 2), (18, 1)] + ([] if tq == 9 else [(20, 2)])
     b = Batch()
     qc = []
@@ -130,9 +133,9 @@ def test_batch_tile_widths_match_oracle(tq):
     terms = workloads.random_pauli_hamiltonian(15, 25, 4, 700)
     prm = list(np.random.default_rng(90).uniform(-1, 1, 15 * 3))
     b.add_vqe(15, 3, terms, prm)
-    b.set_tile(tq)
+    b.set_tile(tq, exact=exact)
     en, amax = b.evaluate()
-    print("tile %d: evaluate %.3f ms" % (tq, b.last_ms()), flush=True)
+    print("tile %d exact %s: evaluate %.3f ms" % (tq, exact, b.last_ms()), flush=True)
     for k, (n, e, w, p, g, be) in enumerate(qc):
         want, amps = orc.qaoa_cost(n, e, w, g, be, want_state=True)
         assert abs(en[k] - want) <= 1e-5 * (1 + sum(w)), (tq, n, en[k], want)
@@ -148,9 +151,11 @@ def test_batch_24_qubits_tile_widths_agree():
     b.add_qaoa(24, e, w, 2, [0.4, 0.9], [0.3, 0.6])
     got = {}
     for tq in (12, 13):
-        b.set_tile(tq)
+        b.set_tile(tq, exact=True)
         got[tq] = b.evaluate()
     assert abs(got[12][0][0] - got[13][0][0]) <= 1e-5 * (1 + sum(w))
+    b.set_tile(12)  # widened: 24 qubits run on 13-qubit tiles, re-planned on a ready batch
+    assert abs(b.evaluate()[0][0] - got[13][0][0]) <= 1e-5 * (1 + sum(w))
 
 
 def test_set_all_params_equals_per_instance_updates():
