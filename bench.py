@@ -177,7 +177,7 @@ def headline(args, L):
         "traffic": (measured_value(os.path.join(ROOT, "profiles", "fused_stage_30q_c32_hbm_traffic.md"),
                                    "traffic_bytes_per_launch") if (nq, prec, args.depth) == (30, "c32", 40) else None),
         "kernel": "k_fused_stage", "bytes_per_launch": stage_bytes, "mean_launch_ms": round(mean_stage_ms, 4),
-        "limiter": "VALU issue (roofline_valu)"}
+        "limiter": "VALU issue and the scalar-load and LDS latency 5 waves per SIMD leave exposed (roofline_valu)"}
 
     # FP32 vector work of the packed program (the matrix-core phases apart)
     cnt = np.zeros(9, dtype=np.uint64)
@@ -486,6 +486,32 @@ def config5(args):
     return out
 
 
+def wide_pass_case(spec):
+    """--wide-pass-case n:p:count[:tile[:exact]]: `count` QAOA MaxCut
+    instances of n qubits and depth p (G(n, 0.5), U(0,1) weights, angles
+    0.3 / 0.7) evaluated three times on the batched engine, so that every
+    k_wide_pass launch covers states of one size: per-pass traffic counters
+    (pass s of an evaluation is launch s mod (p + 1)).  Prints one JSON line."""
+    from ruqu_amd import workloads
+    from ruqu_amd.batch import Batch
+    f = spec.split(":")
+    n, p, count = int(f[0]), int(f[1]), int(f[2])
+    tile = int(f[3]) if len(f) > 3 else None
+    exact = len(f) > 4 and f[4] == "exact"
+    b = Batch()
+    for i in range(count):
+        e, w = workloads.erdos_renyi_graph(n, 0.5, 1000 + i)
+        b.add_qaoa(n, e, w, p, [0.3] * p, [0.7] * p)
+    if tile:
+        b.set_tile(tile, exact=exact)
+    ms = []
+    for _ in range(3):
+        en, _ = b.evaluate()
+        ms.append(b.last_ms())
+    print(json.dumps({"wide_pass_case": spec, "qubits": n, "p": p, "instances": count, "tile": b.tile(),
+                      "exact": exact, "evaluate_ms": [round(x, 3) for x in ms], "energy0": float(en[0])}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -504,12 +530,16 @@ def main():
     ap.add_argument("--qaoa-iters", type=int, default=16)
     ap.add_argument("--qaoa-rescue", type=int, default=6,
                     help="plain updates from the warm angles for instances below the 1%% bar before falling back")
+    ap.add_argument("--wide-pass-case", default=None, metavar="N:P:COUNT[:TILE[:exact]]",
+                    help="only evaluate a uniform batch of QAOA instances (traffic profiles of the batched passes)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU share "
               "(multi-GPU qubit sharding is a design note, not implemented)", file=sys.stderr)
         sys.exit(2)
 
+    if args.wide_pass_case:
+        return wide_pass_case(args.wide_pass_case)
     from ruqu_amd._native import lib
     L = lib()
     h = headline(args, L)
