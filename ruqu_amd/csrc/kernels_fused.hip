@@ -312,12 +312,16 @@ __device__ __forceinline__ void u1f_anti(const Amps<T>& x, Amps<T>& y, T rr, T r
 //   y: matrix / diagonal offset in the pool
 //   c: for a factored 2x2 on a complex-f32 state, its coefficients r and s
 //      (filled in at upload), so the op needs no second, dependent load
+//   tmask: for a factored 2x2, its form (bit 31: unit anti-diagonal) and the
+//      tile-id bit that toggles it; the form of a tile is the parity of
+//      tmask & (blockIdx.x | 1 << 31)
 // Locations: 0 none, LOC_REG + 1 register bit, LOC_THR + 1 thread bit,
 // LOC_OUT + 1 tile-id bit (position = bit of blockIdx.x).
 struct POp {
   uint32_t x, y;
   float c[4];
-  uint32_t pad[2];
+  uint32_t tmask;  // factored 2x2: form bit 31, tile-id bit of the form toggle (0: none)
+  uint32_t pad;
 };
 static_assert(sizeof(POp) == 32, "two records per 64-byte scalar load");
 constexpr int PL_NONE = 0, PL_REG = LOC_REG + 1, PL_THR = LOC_THR + 1, PL_OUT = LOC_OUT + 1;
@@ -369,9 +373,9 @@ __device__ __forceinline__ void apply_one(const Amps<T>& x, Amps<T>& y, const PO
   if (__builtin_expect(kind == (uint32_t)POP_U1F, 1)) {  // cloc/cpos: a tile-id bit that toggles the form
     const int t0 = (w >> 3) & 3u;
     // the toggle is a tile-id bit (never a thread bit): wave-uniform, a
-    // scalar branch.  Integer arithmetic only (cloc == PL_OUT: bits 7 and 8
-    // both set), so the test stays on the scalar unit.
-    const uint32_t anti = ((w >> 5) ^ ((blockIdx.x >> ((w >> 9) & 31u)) & (w >> 7) & (w >> 8))) & 1u;
+    // scalar branch (integer arithmetic only, so the test stays on the
+    // scalar unit)
+    const uint32_t anti = __builtin_popcount(r.tmask & (blockIdx.x | 0x80000000u)) & 1u;
     T rr, ri, sr, si;
     if constexpr (sizeof(T) == 4) {
       rr = r.c[0]; ri = r.c[1]; sr = r.c[2]; si = r.c[3];
@@ -476,8 +480,10 @@ __device__ __forceinline__ void apply_ops(Amps<T>& v, const POp* __restrict__ op
       const u32x16 raw = *reinterpret_cast<const u32x16*>(ops + o);
       p0.x = raw[0];
       p0.y = raw[1];
+      p0.tmask = raw[6];
       p1.x = raw[8];
       p1.y = raw[9];
+      p1.tmask = raw[14];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         p0.c[i] = __uint_as_float(raw[2 + i]);
@@ -1137,6 +1143,7 @@ int pack_ops(const FusedPlan& plan, int mf_min, std::vector<POp>& out, std::vect
         w.x = kind | (t0 << 3) | (t1 << 5) | (cl << 7) | (cp << 9) | (dl0 << 14) | (dp0 << 16) | (dl1 << 22) |
               (dp1 << 24);
         w.y = mat;
+        if (kind == (uint32_t)POP_U1F) w.tmask = (t1 << 31) | (cl == (uint32_t)PL_OUT ? 1u << cp : 0u);
         out.push_back(w);
       };
       auto flush = [&]() {  // apply the pending diagonal as one table record
@@ -1273,7 +1280,7 @@ int pack_ops(const FusedPlan& plan, int mf_min, std::vector<POp>& out, std::vect
         }
       }
       flush();
-      if ((out.size() - (size_t)phases[ph].op_begin) & 1) out.push_back(POp{(uint32_t)POP_NOP, 0u, {0.f, 0.f, 0.f, 0.f}, {0u, 0u}});
+      if ((out.size() - (size_t)phases[ph].op_begin) & 1) out.push_back(POp{(uint32_t)POP_NOP, 0u, {0.f, 0.f, 0.f, 0.f}, 0u, 0u});
       phases[ph].op_end = (int32_t)out.size();
     }
   }
