@@ -145,6 +145,31 @@ def test_batch_tile_widths_match_oracle(tq, exact):
     assert abs(en[-1] - want) <= 1e-5 * (1 + sum(abs(c) for c, _ in terms))
 
 
+def test_batch_widened_tiles_vqe_and_qaoa_match_oracle():
+    """A 21-qubit VQE state (its Pauli energies folded over 13-qubit tiles)
+    and a 22-qubit QAOA state widened to 13-qubit tiles beside a 16-qubit
+    state on 12-qubit tiles: every pass launches once per width."""
+    b = Batch()
+    qc = []
+    for k, (n, p) in enumerate([(22, 2), (16, 3)]):
+        e, w = workloads.erdos_renyi_graph(n, 0.3, 900 + k)
+        rng = np.random.default_rng(95 + k)
+        qc.append((n, e, w, p, list(rng.uniform(0, math.pi, p)), list(rng.uniform(0, math.pi, p))))
+        b.add_qaoa(*qc[-1])
+    terms = workloads.random_pauli_hamiltonian(21, 12, 4, 901)
+    prm = list(np.random.default_rng(96).uniform(-1, 1, 21 * 2))
+    b.add_vqe(21, 2, terms, prm)
+    b.set_tile(12)
+    en, amax = b.evaluate()
+    for k, (n, e, w, p, g, be) in enumerate(qc):
+        want, amps = orc.qaoa_cost(n, e, w, g, be, want_state=True)
+        assert abs(en[k] - want) <= 1e-5 * (1 + sum(w)), (n, en[k], want)
+        pr = np.abs(amps) ** 2
+        assert pr[amax[k]] >= pr.max() * (1 - 1e-4)
+    want = orc.energy(21, workloads.ry_cz_ansatz(21, 2, prm).to_array(), terms)
+    assert abs(en[-1] - want) <= 1e-5 * (1 + sum(abs(c) for c, _ in terms)), (en[-1], want)
+
+
 def test_batch_24_qubits_tile_widths_agree():
     e, w = workloads.erdos_renyi_graph(24, 0.3, 800)
     b = Batch()
