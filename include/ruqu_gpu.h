@@ -321,7 +321,7 @@ int rq_argmax_probability(const rq_state* s, uint64_t* index, double* prob);
 int rq_expectation_zmasks(const rq_state* s, const uint64_t* masks, size_t n, double* out);
 
 /* ---- batched QAOA / VQE (configs with thousands of small instances) ---- */
-/* Many independent instances (2..24 qubits QAOA, 2..16 qubits VQE with the
+/* Many independent instances (2..24 qubits, QAOA MaxCut or VQE with the
  * Ry-CZ ansatz) in complex-f32 with FP64 reductions, evaluated by one
  * persistent kernel per optimizer round; optimizer steps follow run_qaoa
  * (qaoa.rs:217-308) and run_vqe (vqe.rs:148-216) per instance. */

@@ -694,7 +694,8 @@ int rq_batch_add_vqe(rq_batch* h, uint32_t n, uint32_t layers, const double* coe
   Batch* b = (Batch*)h;
   if (!b || (nterms && (!coeffs || !offsets))) return fail(RQ_ERR_ARGUMENT, "null pointer");
   if (b->ready) return fail(RQ_ERR_ARGUMENT, "batch already finalized");
-  if (n < 2 || n > 16) return fail(RQ_ERR_QUBIT_LIMIT, "batched VQE instances take 2..16 qubits");
+  if (n < 2 || n > (uint32_t)BATCH_MAX_QUBITS)
+    return fail(RQ_ERR_QUBIT_LIMIT, "batched VQE instances take 2.." + std::to_string(BATCH_MAX_QUBITS) + " qubits");
   BInst in{};
   in.kind = BK_VQE;
   in.n = (int)n;

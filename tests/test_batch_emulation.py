@@ -145,6 +145,38 @@ def test_batch_widens_tiles_that_would_cut_lines():
         assert pr[am[k]] >= pr.max() * (1 - 1e-4)
 
 
+def test_batch_widened_vqe_state_on_cpu():
+    """A VQE state past the config's 16 qubits: with base tile 10 an 18-qubit
+    state gets 11-qubit tiles (runs of 16 amplitudes in the second tiling)
+    beside a 14-qubit state on 10-qubit tiles; its Pauli energy is folded over
+    the wider tiles.  Energies equal the oracle's, as with uniform tiles."""
+    b = Batch()
+    cases = []
+    for k, n in enumerate((14, 18)):
+        terms = workloads.random_pauli_hamiltonian(n, 8, 4, 700 + k)
+        prm = list(np.random.default_rng(70 + k).uniform(-1, 1, 2 * n))
+        cases.append((n, terms, prm))
+        b.add_vqe(n, 2, terms, prm)
+    b.set_tile(10)
+    assert list(batch_emu.data(b)["insts"]["tq"]) == [10, 11]
+    en, _ = batch_emu.evaluate(b)
+    b.set_tile(10, exact=True)
+    en_exact, _ = batch_emu.evaluate(b)
+    for k, (n, terms, prm) in enumerate(cases):
+        want = orc.energy(n, workloads.ry_cz_ansatz(n, 2, prm).to_array(), terms)
+        tol = 1e-5 * (1 + sum(abs(c) for c, _ in terms))
+        assert abs(en[k] - want) <= tol and abs(en_exact[k] - want) <= tol, (n, en[k], en_exact[k], want)
+
+
+def test_batch_vqe_qubit_range():
+    b = Batch()
+    with pytest.raises(Exception):
+        b.add_vqe(25, 1, workloads.random_pauli_hamiltonian(4, 2, 2, 1))
+    with pytest.raises(Exception):
+        b.add_vqe(1, 1, [(1.0, [(0, 3)])])
+    b.add_vqe(24, 1, workloads.random_pauli_hamiltonian(24, 2, 2, 1))
+
+
 def test_wide_rotation_factoring_modes():
     """Wide states rotate with the larger of cos / sin factored out of each
     2x2 (4 forms: Rx or Ry, cos or sin factored) and one uniform scale per
