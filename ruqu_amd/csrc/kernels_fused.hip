@@ -14,6 +14,7 @@
 //
 // Arithmetic: FP contraction is allowed here (fused mode), so results differ
 // from the per-gate kernels at the rounding level.
+#include <array>
 #include <cstddef>
 #include <cstring>
 #include <utility>
@@ -933,6 +934,16 @@ struct PendingDiag {
       if (!eq(x, one)) return false;
     return true;
   }
+  // g constant over the 4 bits (its entries all equal): the constant moves
+  // into d[0] and g becomes 1; false when g couples bits.
+  bool separate() {
+    for (int k = 1; k < FUSE_R; ++k)
+      if (!eq(g[k], g[0])) return false;
+    d[0][0] = mul(d[0][0], g[0]);
+    d[0][1] = mul(d[0][1], g[0]);
+    for (auto& x : g) x = rq_c64{1.0, 0.0};
+    return true;
+  }
   rq_c64 entry(int k) const {
     rq_c64 v = g[k];
     for (int b = 0; b < FUSE_R_BITS; ++b) v = mul(v, d[b][(k >> b) & 1]);
@@ -1042,6 +1053,15 @@ bool touches_reg_bit(const FOp& x, int b) {
 // condition on a bit it depends on, an op on a bit it couples to others) and
 // at the end of the phase.
 //
+// Carried factors: a phase whose pending diagonal holds only per-bit factors
+// (g constant) applies no table at its end.  The factor of register bit b
+// (global qubit q) is carried to the next phase in which q is a register bit,
+// where it starts that phase's pending factor of q (a matrix-core phase folds
+// it into the columns of its unitaries).  In between, every op that involves
+// q acts on it as a control or a diagonal bit, and those commute with a
+// diagonal on q.  A qubit's last register phase of the program applies its
+// factor there.
+//
 // Matrix-core phases (mf_min >= 0, complex f32): a phase whose ops cost at
 // least mf_min VALU instructions per wave and whose conditions are register
 // bits or uniform bits (MfBits) gets no records: its composed unitaries go to
@@ -1079,6 +1099,13 @@ int pack_ops(const FusedPlan& plan, int mf_min, std::vector<POp>& out, std::vect
       }
     }
   }
+  // last phase in which each global qubit is a register bit; the factors carried between phases
+  std::vector<int> last_reg(64, -1);  // global qubits are < 64
+  for (const FStage& st : plan.stages)
+    for (int ph = st.phase0; ph < st.phase0 + st.nphase; ++ph)
+      for (int b = 0; b < FUSE_R_BITS; ++b) last_reg[st.S[plan.phases[ph].regt[b]]] = ph;
+  std::vector<std::array<rq_c64, 2>> carry(64, {rq_c64{1.0, 0.0}, rq_c64{1.0, 0.0}});
+  auto qubit_of = [&](const FStage& st, int ph, int b) { return st.S[plan.phases[ph].regt[b]]; };
   std::vector<int> fold_into(plan.ops.size(), -1);  // op -> index of the 2x2 it merges into
   std::vector<int> folded_by(plan.ops.size(), -1);  // 2x2 -> the op merged into it
   for (size_t pi = 0; pi < plan.phases.size(); ++pi) {
@@ -1124,12 +1151,18 @@ int pack_ops(const FusedPlan& plan, int mf_min, std::vector<POp>& out, std::vect
         for (uint32_t vi = 0; vi < nv; ++vi) {
           rq_c64 U[FUSE_R][FUSE_R];
           mf_compose(plan, plan.phases[ph], st.m - FUSE_R_BITS, tile_bit, mfb[ph], vi, U);
+          for (int col = 0; col < FUSE_R; ++col) {  // the carried factors act first: U diag(carry)
+            rq_c64 f{1.0, 0.0};
+            for (int b = 0; b < FUSE_R_BITS; ++b) f = PendingDiag::mul(f, carry[qubit_of(st, ph, b)][(col >> b) & 1]);
+            for (int r = 0; r < FUSE_R; ++r) U[r][col] = PendingDiag::mul(U[r][col], f);
+          }
           mf_operands(U, w.data());
           for (int f = 0; f < MF_FLOATS; ++f) {
             rq_c64& e = pool[off + vi * (MF_FLOATS / 2) + f / 2];
             (f & 1 ? e.im : e.re) = (double)w[f];
           }
         }
+        for (int b = 0; b < FUSE_R_BITS; ++b) carry[qubit_of(st, ph, b)] = {rq_c64{1.0, 0.0}, rq_c64{1.0, 0.0}};
         phases[ph].op_end = (int32_t)out.size();
         phases[ph].mf_off = (int32_t)off;
         phases[ph].mf_sel = mfb[ph].sel();
@@ -1137,6 +1170,11 @@ int pack_ops(const FusedPlan& plan, int mf_min, std::vector<POp>& out, std::vect
         continue;
       }
       PendingDiag pd;
+      for (int b = 0; b < FUSE_R_BITS; ++b) {  // factors carried in from earlier phases
+        auto& c = carry[qubit_of(st, ph, b)];
+        pd.set_bit(b, c.data());
+        c = {rq_c64{1.0, 0.0}, rq_c64{1.0, 0.0}};
+      }
       auto emit = [&](uint32_t kind, uint32_t t0, uint32_t t1, uint32_t cl, uint32_t cp, uint32_t dl0, uint32_t dp0,
                       uint32_t dl1, uint32_t dp1, uint32_t mat) {
         POp w{};
@@ -1277,6 +1315,14 @@ int pack_ops(const FusedPlan& plan, int mf_min, std::vector<POp>& out, std::vect
             if (dl0 == PL_REG && dl1 == PL_REG) pd.d2((int)dp0, (int)dp1, &pool[mat]);  // absorbed
             else emit((uint32_t)FOP_D2, 0, 0, 0, 0, dl0, dp0, dl1, dp1, mat);
             break;
+        }
+      }
+      if (pd.separate()) {  // per-bit factors only: carry those of qubits with a later register phase
+        for (int b = 0; b < FUSE_R_BITS; ++b) {
+          const int q = qubit_of(st, ph, b);
+          if (last_reg[q] == ph) continue;
+          carry[q] = {pd.d[b][0], pd.d[b][1]};
+          pd.clear_bit(b);
         }
       }
       flush();
