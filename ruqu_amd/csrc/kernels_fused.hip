@@ -206,6 +206,29 @@ __device__ __forceinline__ void cx_sel(const Amps<T>& x, Amps<T>& y, uint64_t la
   }
 }
 
+// The same permutations after a relative phase rho on the amplitudes whose
+// register bit B is set (the pending factor of the target bit moved in front
+// of the CNOT, so that it leaves no coupling behind).
+template <int B, int C, class T>
+__device__ __forceinline__ void cx_rc_ph(const Amps<T>& x, Amps<T>& y, Cx<T> rho) {
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) {
+    const int src = ((k >> C) & 1) ? k ^ (1 << B) : k;
+    y[k] = ((src >> B) & 1) ? cmul(rho, x[src]) : fresh(x[src]);
+  }
+}
+template <int B, class T>
+__device__ __forceinline__ void cx_sel_ph(const Amps<T>& x, Amps<T>& y, uint64_t lanes, Cx<T> rho) {
+#pragma unroll
+  for (int k = 0; k < FUSE_R; ++k) {
+    if (k & (1 << B)) continue;
+    const int k1 = k | (1 << B);
+    const Cx<T> a = x[k], b = cmul(rho, x[k1]);
+    y[k] = Cx<T>{pick(a.re, b.re, lanes), pick(a.im, b.im, lanes)};
+    y[k1] = Cx<T>{pick(b.re, a.re, lanes), pick(b.im, a.im, lanes)};
+  }
+}
+
 // CZ: negate where register bits P0 and P1 are set; where register bit P is
 // set (sign applied: the other bit's value << 31); everywhere (sign).
 template <int P0, int P1, class T>
@@ -405,6 +428,18 @@ __device__ __forceinline__ void apply_one(const Amps<T>& x, Amps<T>& y, const PO
     }
     case FOP_CX: {
       const int t0 = (w >> 3) & 3u, cl = (w >> 7) & 3u, cp = (w >> 9) & 31u;
+      if ((w >> 5) & 1u) {  // t1 = 1: relative phase on the target's set half first (register or lane control)
+        Cx<T> rho;
+        if constexpr (sizeof(T) == 4) rho = Cx<T>{r.c[0], r.c[1]};
+        else rho = mp[0];
+        if (cl == PL_REG) {
+          const int p0 = t0, p1 = cp;
+          RQ_P2(cx_rc_ph, x, y, rho)
+        } else {
+          RQ_P1(cx_sel_ph, t0, x, y, __builtin_amdgcn_ballot_w64(loc_val(cl, cp, tid) != 0u), rho)
+        }
+        break;
+      }
       if (cl == PL_REG) {
         const int p0 = t0, p1 = cp;
         RQ_P2(cx_rc, x, y)
@@ -1295,12 +1330,27 @@ int pack_ops(const FusedPlan& plan, int mf_min, std::vector<POp>& out, std::vect
             emit((uint32_t)FOP_U2, (uint32_t)op.t0, (uint32_t)op.t1, 0, 0, 0, 0, 0, 0, nm);
             break;
           }
-          case FOP_CX:
+          case FOP_CX: {
+            // A controlled CNOT after a pending factor (d0, d1) on its target: the relative phase
+            // d1 / d0 goes into the record (applied to the target's set half before the
+            // permutation) and d0, the same for both values of the bit, stays pending.  Moved
+            // through the CNOT, (d0, d1) would couple the target to the control (a table flush).
+            uint32_t phase = 0;
+            if (cl != PL_NONE && pd.free_of(b) && !PendingDiag::eq(pd.d[b][0], pd.d[b][1]) &&
+                (pd.d[b][0].re != 0.0 || pd.d[b][0].im != 0.0)) {
+              const rq_c64 d0 = pd.d[b][0], d1 = pd.d[b][1];
+              const double n = d0.re * d0.re + d0.im * d0.im;
+              mat = (uint32_t)pool.size();
+              pool.push_back(rq_c64{(d1.re * d0.re + d1.im * d0.im) / n, (d1.im * d0.re - d1.re * d0.im) / n});
+              pd.d[b][1] = d0;
+              phase = 1;
+            }
             if (cl == PL_REG) pd.cx_reg((int)cp, b);
             else if (cl == PL_NONE) pd.cx_all(b);
             else if (!pd.flip_invariant(b)) flush();  // a per-lane / per-tile condition
-            emit((uint32_t)FOP_CX, (uint32_t)b, 0, cl, cp, 0, 0, 0, 0, mat);
+            emit((uint32_t)FOP_CX, (uint32_t)b, phase, cl, cp, 0, 0, 0, 0, mat);
             break;
+          }
           case FOP_CZ:
             // Absorbed into the pending diagonal unless a later 2x2 / 4x4 of the phase acts on one of its
             // bits: the coupling would force a table flush (64 instructions) where the CZ record costs 32.
@@ -1383,11 +1433,15 @@ int upload_plan_t(State& s, const FusedPlan& plan, FusedProgram* fp) {
   std::memcpy(host.data() + o_ph, phases.data(), nph * sizeof(FPhase));
   if constexpr (sizeof(T) == 4) {  // factored 2x2 records carry their f32 coefficients inline
     for (POp& p : pops)
-      if ((p.x & 7u) == (uint32_t)POP_U1F)
+      if ((p.x & 7u) == (uint32_t)POP_U1F) {
         for (int k = 0; k < 2; ++k) {
           p.c[2 * k] = (float)poolx[p.y + k].re;
           p.c[2 * k + 1] = (float)poolx[p.y + k].im;
         }
+      } else if ((p.x & 7u) == (uint32_t)FOP_CX && ((p.x >> 5) & 1u)) {  // the relative phase of a CNOT
+        p.c[0] = (float)poolx[p.y].re;
+        p.c[1] = (float)poolx[p.y].im;
+      }
   }
   std::memcpy(host.data() + o_op, pops.data(), nop * sizeof(POp));
   Cx<T>* pp = reinterpret_cast<Cx<T>*>(host.data() + o_pool);
@@ -1468,9 +1522,11 @@ void host_apply_one(Cx<T> (&x)[FUSE_R], uint32_t w, const Cx<T>* mp, uint32_t ti
     }
   } else if (kind == FOP_CX) {
     const int bb = 1 << t0;
+    Cx<T> xp[FUSE_R];
+    for (int k = 0; k < FUSE_R; ++k) xp[k] = ((t1 & 1) && (k & bb)) ? mul(mp[0], x[k]) : x[k];
     for (int k = 0; k < FUSE_R; ++k) {
       const bool on = cl == PL_NONE || (cl == PL_REG ? ((k >> cp) & 1) : lv(cl, cp));
-      if (on) y[k] = x[k ^ bb];
+      y[k] = on ? xp[k ^ bb] : xp[k];
     }
   } else if (kind == FOP_CZ) {
     for (int k = 0; k < FUSE_R; ++k)
