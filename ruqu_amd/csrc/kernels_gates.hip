@@ -74,6 +74,18 @@ __device__ __forceinline__ uint64_t xcd_block(uint32_t C) {
   return ((r / C) * 8u + x) * C + (r % C);
 }
 
+// A zero computed from a loaded value, opaque to the compiler.  The second
+// stream's addresses of the two-stream kernels are offset by it, so each
+// thread issues its partner loads only when its first loads have returned:
+// the two loads of a pair (2^q amplitudes apart) are then not in flight
+// together.  Issued together, pairs 16 MB, 256 MB and 4 GB apart ran at
+// 5.7-5.8 TB/s on MI355X; issued in turn, with the chunks below, every
+// distance from 1 KB to 4 GB runs at 6.46-6.84 TB/s.
+__device__ __forceinline__ uint32_t zero_after(uint32_t v) {
+  asm volatile("v_and_b32 %0, 0, %0" : "+v"(v));
+  return v;
+}
+
 // Insert a zero bit at position q of p.
 __device__ __forceinline__ uint64_t ins0(uint64_t p, uint32_t q) {
   uint64_t lo = p & ((1ull << q) - 1);
@@ -87,6 +99,7 @@ __global__ __launch_bounds__(BLOCK) void k_1q_f32_v2(float4* __restrict__ a, uin
   const uint64_t g0 = xcd_block(chunk) * U * BLOCK + threadIdx.x;
   float4 x[U], y[U];
   uint64_t ii[U], jj[U];
+  uint32_t z = 0;
 #pragma unroll
   for (int k = 0; k < U; ++k) {
     uint64_t g = g0 + (uint64_t)k * BLOCK;
@@ -95,8 +108,14 @@ __global__ __launch_bounds__(BLOCK) void k_1q_f32_v2(float4* __restrict__ a, uin
       ii[k] = i >> 1;
       jj[k] = (i + (1ull << q)) >> 1;
       x[k] = ldg<NT>(a + ii[k]);
-      y[k] = ldg<NT>(a + jj[k]);
+      z = __float_as_uint(x[k].x);
     }
+  }
+  z = zero_after(z);
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    uint64_t g = g0 + (uint64_t)k * BLOCK;
+    if (g < ngroups) y[k] = ldg<NT>(a + jj[k] + z);
   }
 #pragma unroll
   for (int k = 0; k < U; ++k) {
@@ -178,14 +197,21 @@ __global__ __launch_bounds__(BLOCK) void k_1q_f64(double2* __restrict__ a, uint3
   const uint64_t g0 = xcd_block(chunk) * U * BLOCK + threadIdx.x;
   double2 x[U], y[U];
   uint64_t ii[U];
+  uint32_t z = 0;
 #pragma unroll
   for (int k = 0; k < U; ++k) {
     uint64_t g = g0 + (uint64_t)k * BLOCK;
     if (g < npairs) {
       ii[k] = ins0(g, q);
       x[k] = ldg<NT>(a + ii[k]);
-      y[k] = ldg<NT>(a + ii[k] + (1ull << q));
+      z = (uint32_t)__double_as_longlong(x[k].x);
     }
+  }
+  z = zero_after(z);
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    uint64_t g = g0 + (uint64_t)k * BLOCK;
+    if (g < npairs) y[k] = ldg<NT>(a + ii[k] + (1ull << q) + z);
   }
 #pragma unroll
   for (int k = 0; k < U; ++k) {
@@ -695,17 +721,20 @@ M4<T> to_m4(const rq_c64* m) {
 }
 
 // Block chunk of the two-stream 1q kernels by log2 of the distance between the
-// streams in bytes (2^q * amplitude size): the best of 0 (linear order), 64,
-// 128, 256, 512 and 1024 blocks per distance, measured on MI355X with a
-// 30-qubit complex-f32 state (non-temporal access, one group per thread).
-// 1 KB .. 8 KB apart: 5.93-6.16 TB/s linear, 6.36-6.40 with 64.
+// streams in bytes (2^q * amplitude size), measured on MI355X with a 30-qubit
+// complex-f32 state (non-temporal access, one group per thread, partner loads
+// issued after the first ones returned) over chunks 0 (linear order) and
+// 32 .. 2048: 32 blocks (128 KB of each stream per XCD at a time) is best up
+// to 256 KB apart; from 8 MB on, 32 and 64 alternate with period 4 in log2 of
+// the distance (32 loses 0.6 TB/s at 8 MB, 128 MB and 2 GB, 64 as much at
+// 16 MB, 256 MB and 4 GB).
 constexpr uint16_t XCD_CHUNK_BY_LOG2_DIST[64] = {
-    64,  64,  64,  64,  64,  64,  64,  64,  64,  64,     // 0..9
-    64,  64,  64,  64,  256, 64,  64,  128, 64,  128,    // 10..19: 1 KB 64, 16 KB 256, 32-64 KB 64, 128 KB 128
-    0,   256, 128, 256, 1024, 64, 1024, 256, 128, 64,    // 20..29
-    128, 256, 128, 128, 128, 128, 128, 128, 128, 128,    // 30..39
-    128, 128, 128, 128, 128, 128, 128, 128, 128, 128, 128, 128, 128, 128,
-    128, 128, 128, 128, 128, 128, 128, 128, 128, 128};
+    32,  32,  32,  32,  32,  32,  32,  32,  32,  32,     // 0..9
+    32,  32,  32,  32,  32,  32,  32,  32,  32,  128,    // 10..19: 512 KB 128
+    32,  32,  32,  64,  32,  32,  64,  64,  32,  64,     // 20..29
+    64,  64,  32,  64,  64,  64,  32,  64,  64,  64,     // 30..39
+    32,  64,  64,  64,  32,  64,  64,  64,  32,  64, 64, 64, 32, 64,
+    64,  64,  32,  64,  64,  64,  32,  64,  64,  64};
 
 // The chunk for a two-stream launch of nb workgroups whose streams are
 // 2^log2d bytes apart: the tuned value or the table, and 0 when nb is not a
