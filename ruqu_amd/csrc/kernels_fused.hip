@@ -73,22 +73,6 @@ __device__ __forceinline__ uint32_t lane_lds_addr(uint32_t t) {
   return a;
 }
 
-template <class T>
-__device__ __forceinline__ void load_amp(const T* __restrict__ a, uint64_t g, Cx<T>& v) {
-  using V = typename Vec2<T>::type;
-  V x = reinterpret_cast<const V*>(a)[g];
-  v.re = x.x;
-  v.im = x.y;
-}
-template <class T>
-__device__ __forceinline__ void store_amp(T* __restrict__ a, uint64_t g, const Cx<T>& v) {
-  using V = typename Vec2<T>::type;
-  V x;
-  x.x = v.re;
-  x.y = v.im;
-  reinterpret_cast<V*>(a)[g] = x;
-}
-
 // Every op reads the 16 amplitudes from x and writes all 16 to y, a distinct
 // register set.  apply_ops alternates the two sets (x -> y, then y -> x), so
 // an op's results land in fresh registers and no copy back is needed between
@@ -678,15 +662,23 @@ __device__ __forceinline__ uint64_t greg_of(const Map& mp) {
          ((K & 8) ? mp.greg[3] : 0ull);
 }
 
+// HBM accesses of a mapping: the thread's part and the (uniform) register
+// part of an index have disjoint bits, so the address is the thread's
+// pointer plus a uniform offset (one 64-bit add per access).
 template <int K, class T>
 struct Unroll {
-  static __device__ __forceinline__ void gather_g(Cx<T> (&v)[FUSE_R], const T* a, const Map& mp) {
-    Unroll<K - 1, T>::gather_g(v, a, mp);
-    load_amp(a, mp.gthr | greg_of<K - 1>(mp), v[K - 1]);
+  using V = typename Vec2<T>::type;
+  static __device__ __forceinline__ void gather_g(Cx<T> (&v)[FUSE_R], const V* at, const Map& mp) {
+    Unroll<K - 1, T>::gather_g(v, at, mp);
+    const V x = at[greg_of<K - 1>(mp)];
+    v[K - 1] = Cx<T>{x.x, x.y};
   }
-  static __device__ __forceinline__ void scatter_g(const Cx<T> (&v)[FUSE_R], T* a, const Map& mp) {
-    Unroll<K - 1, T>::scatter_g(v, a, mp);
-    store_amp(a, mp.gthr | greg_of<K - 1>(mp), v[K - 1]);
+  static __device__ __forceinline__ void scatter_g(const Cx<T> (&v)[FUSE_R], V* at, const Map& mp) {
+    Unroll<K - 1, T>::scatter_g(v, at, mp);
+    V x;
+    x.x = v[K - 1].re;
+    x.y = v[K - 1].im;
+    at[greg_of<K - 1>(mp)] = x;
   }
   static __device__ __forceinline__ void gather_l(Cx<T> (&v)[FUSE_R], const Cx<T>* lds, const Map& mp,
                                                   uint32_t lt) {
@@ -701,8 +693,9 @@ struct Unroll {
 };
 template <class T>
 struct Unroll<0, T> {
-  static __device__ __forceinline__ void gather_g(Cx<T> (&)[FUSE_R], const T*, const Map&) {}
-  static __device__ __forceinline__ void scatter_g(const Cx<T> (&)[FUSE_R], T*, const Map&) {}
+  using V = typename Vec2<T>::type;
+  static __device__ __forceinline__ void gather_g(Cx<T> (&)[FUSE_R], const V*, const Map&) {}
+  static __device__ __forceinline__ void scatter_g(const Cx<T> (&)[FUSE_R], V*, const Map&) {}
   static __device__ __forceinline__ void gather_l(Cx<T> (&)[FUSE_R], const Cx<T>*, const Map&, uint32_t) {}
   static __device__ __forceinline__ void scatter_l(const Cx<T> (&)[FUSE_R], Cx<T>*, const Map&, uint32_t) {}
 };
@@ -714,6 +707,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   // A static array (the kernel's only LDS) sits at LDS address 0, so the
   // compiler folds its base into the addresses.
   __shared__ Cx<T> lds[1 << FUSE_MAX_M];
+  using V2 = typename Vec2<T>::type;
   const FStage& st = *stp;
   const uint32_t tid = threadIdx.x;
   uint64_t outer = 0;
@@ -739,10 +733,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     const Map mp = make_map(ph, tid, outer);
     if (p == 0) {
       if (ph.direct()) {
-        Unroll<FUSE_R, T>::gather_g(v, a, mp);
+        Unroll<FUSE_R, T>::gather_g(v, reinterpret_cast<const V2*>(a) + mp.gthr, mp);
       } else {
         const Map nat = make_natural_map(st, tid, outer);
-        Unroll<FUSE_R, T>::gather_g(v, a, nat);
+        Unroll<FUSE_R, T>::gather_g(v, reinterpret_cast<const V2*>(a) + nat.gthr, nat);
         Unroll<FUSE_R, T>::scatter_l(v, lds, nat, lane_lds_addr<T>(nat.tthr));
         __syncthreads();
         Unroll<FUSE_R, T>::gather_l(v, lds, mp, lane_lds_addr<T>(mp.tthr));
@@ -762,13 +756,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     __builtin_amdgcn_s_setprio(3);
     if (p == np - 1) {
       if (ph.direct()) {
-        Unroll<FUSE_R, T>::scatter_g(v, a, mp);
+        Unroll<FUSE_R, T>::scatter_g(v, reinterpret_cast<V2*>(a) + mp.gthr, mp);
       } else {
         const Map nat = make_natural_map(st, tid, outer);
         Unroll<FUSE_R, T>::scatter_l(v, lds, mp, lane_lds_addr<T>(mp.tthr));
         __syncthreads();
         Unroll<FUSE_R, T>::gather_l(v, lds, nat, lane_lds_addr<T>(nat.tthr));
-        Unroll<FUSE_R, T>::scatter_g(v, a, nat);
+        Unroll<FUSE_R, T>::scatter_g(v, reinterpret_cast<V2*>(a) + nat.gthr, nat);
       }
     } else {
       Unroll<FUSE_R, T>::scatter_l(v, lds, mp, lane_lds_addr<T>(mp.tthr));
