@@ -352,14 +352,29 @@ struct WideCtx {
   __device__ __forceinline__ int qubit_of(int bit) const { return tb ? TQ + bit - lowb : bit; }
 };
 
+// The thread's part of a window's tile indices and m << lo have disjoint
+// bits, so the LDS slot of register m is a per-thread base plus a uniform
+// offset (one add per access; the offsets are scalar).
+template <int R>
+__device__ __forceinline__ uint32_t win_slot_base(uint32_t tid, int lo) {
+  const uint32_t t = (tid & ((1u << lo) - 1u)) | ((tid >> lo) << (lo + R));
+  return t + (t >> R);
+}
+template <int R>
+__device__ __forceinline__ uint32_t win_slot_off(int lo, int m) {
+  const uint32_t k = (uint32_t)m << lo;
+  return k + (k >> R);
+}
+
 template <int TQ>
 __device__ __forceinline__ void wide_transpose(float2 (&v)[1 << (TQ - 8)], int from, int to, float2* lds) {
   constexpr int R = TQ - 8, NR = 1 << R;
+  const uint32_t bf = win_slot_base<R>(threadIdx.x, from), bt = win_slot_base<R>(threadIdx.x, to);
 #pragma unroll
-  for (int m = 0; m < NR; ++m) lds[lds_slot(win_index<R>(threadIdx.x, from, m), R)] = v[m];
+  for (int m = 0; m < NR; ++m) lds[bf + win_slot_off<R>(from, m)] = v[m];
   __syncthreads();
 #pragma unroll
-  for (int m = 0; m < NR; ++m) v[m] = lds[lds_slot(win_index<R>(threadIdx.x, to, m), R)];
+  for (int m = 0; m < NR; ++m) v[m] = lds[bt + win_slot_off<R>(to, m)];
   __syncthreads();
 }
 
