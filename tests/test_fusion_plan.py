@@ -166,3 +166,38 @@ def test_packed_records_export():
     assert lib().rq_fusion_packed_records(n, 1, ptr(arr), len(arr), 10, 4, ptr(words), ptr(phase), k - 1,
                                           C.byref(small)) != 0
     assert small.value == k
+
+
+@pytest.mark.parametrize("prec,thr", [("c64", -1), ("c32", -1), ("c32", 0)])
+def test_carried_factors_and_cnot_phases(mf_threshold, prec, thr):
+    """Pending per-qubit factors cross phase and stage boundaries (applied in
+    the qubit's next register phase, folded into matrix-core unitaries, or
+    applied in its last register phase), and a controlled CNOT after a pending
+    factor on its target carries the relative phase in its record.  Deep
+    circuits on small tiles (many stages and phases) equal the oracle, and
+    table records are rare."""
+    mf_threshold(thr)
+    n, m, c = 13, (10 if thr == 0 else 8), 3  # matrix-core phases need 6 thread bits
+    arr = workloads.random_circuit(n, 30, 44).to_array()
+    if thr == 0:
+        assert packed_info(n, arr, m, c, prec)[1] > 0
+    want, _, _ = orc.run_circuit(n, arr)
+    tol = 1e-12 if prec == "c64" else 2e-5
+    assert max_abs(emulate(n, arr, m, c, prec), want) < tol
+    p = 0 if prec == "c64" else 1
+    cnt = np.zeros(9, dtype=np.uint64)
+    assert lib().rq_fusion_packed_kinds(n, p, ptr(arr), len(arr), m, c, ptr(cnt)) == 0
+    _, phases, _ = plan_info(n, arr, m, c)
+    assert int(cnt[8]) * 4 < phases  # tables: far fewer than phases
+    cap = 1 << 16
+    words = np.zeros(cap, dtype=np.uint32)
+    phase = np.zeros(cap, dtype=np.int32)
+    count = C.c_size_t()
+    assert lib().rq_fusion_packed_records(n, p, ptr(arr), len(arr), m, c, ptr(words), ptr(phase), cap,
+                                          C.byref(count)) == 0
+    w = words[:count.value]
+    cx_phase = ((w & 7) == 4) & (((w >> 5) & 3) == 1)
+    assert int(cx_phase.sum()) > 0
+    mixed = mixed_circuit(n, 8, 4242).to_array()
+    want, _, _ = orc.run_circuit(n, mixed)
+    assert max_abs(emulate(n, mixed, m, c, prec), want) < tol
