@@ -16,7 +16,9 @@ The same line carries every workload of the scope table:
                     bound the stage actually sits on;
   per_gate_1q_GBps  the exact single-qubit gate kernel, GB/s for every target;
   f32_error_vs_f64  the complex-f32 fused state against the exact complex-f64
-                    path (bit-identical to the FP64 oracle) at 30 qubits;
+                    path (bit-identical to the FP64 oracle) at 30 qubits, and
+                    (exact_c32) the unfused complex-f32 path, one exact kernel
+                    per gate: its gate-apps/s and its error;
   dense_blocks      the circuit as dense 5-qubit blocks on the matrix cores;
   c64_32q           the same generator at 32 qubits complex f64, depth 20;
   cpu_baseline      the reference's fastest CPU path: simd.rs's AVX2 +
@@ -228,23 +230,40 @@ def headline(args, L):
         exact_s = time.perf_counter() - t1
         a64 = st64.state_vector()
         del st64
-        mx, d2, n64, n32 = 0.0, 0.0, 0.0, 0.0
-        ip = 0j
-        ch = 1 << 24
-        for o in range(0, n, ch):
-            x = a32[o:o + ch].astype(np.complex128)
-            y = a64[o:o + ch]
-            d = x - y
-            mx = max(mx, float(np.max(np.abs(d))))
-            d2 += float(np.vdot(d, d).real)
-            n64 += float(np.vdot(y, y).real)
-            n32 += float(np.vdot(x, x).real)
-            ip += complex(np.vdot(y, x))
-        del a32, a64
-        err = {"max_abs": mx, "rel_l2": (d2 / n64) ** 0.5, "one_minus_fidelity": 1.0 - abs(ip) ** 2 / (n32 * n64),
-               "exact_f64_gate_apps_per_s": round(ngates / exact_s, 1),
-               "note": "complex-f32 fused run vs the exact per-gate complex-f64 run of the same circuit on the "
-                       "device (that path is bit-identical to the FP64 CPU oracle wherever the oracle runs)"}
+        # the unfused complex-f32 path (the reference's QuantumStateF32 arithmetic, one kernel per gate)
+        t2 = time.perf_counter()
+        rc = L.rq_run_circuit(st.handle, ptr(arr), ngates, None, 0, None, 0, C.byref(nout), 0, C.byref(met))
+        if rc:
+            raise RuntimeError(L.rq_last_error().decode())
+        exact32_s = time.perf_counter() - t2
+        exact32_dev_ms = met.device_time_ms
+        a32x = st.state_vector()
+
+        def err_stats(a_32, a_64):
+            mx, d2, n64, n32 = 0.0, 0.0, 0.0, 0.0
+            ip = 0j
+            ch = 1 << 24
+            for o in range(0, n, ch):
+                x = a_32[o:o + ch].astype(np.complex128)
+                y = a_64[o:o + ch]
+                d = x - y
+                mx = max(mx, float(np.max(np.abs(d))))
+                d2 += float(np.vdot(d, d).real)
+                n64 += float(np.vdot(y, y).real)
+                n32 += float(np.vdot(x, x).real)
+                ip += complex(np.vdot(y, x))
+            return {"max_abs": mx, "rel_l2": (d2 / n64) ** 0.5, "one_minus_fidelity": 1.0 - abs(ip) ** 2 / (n32 * n64)}
+
+        err = err_stats(a32, a64)
+        err["exact_f64_gate_apps_per_s"] = round(ngates / exact_s, 1)
+        err["note"] = ("complex-f32 fused run vs the exact per-gate complex-f64 run of the same circuit on the "
+                       "device (that path is bit-identical to the FP64 CPU oracle wherever the oracle runs)")
+        e32 = err_stats(a32x, a64)
+        e32.update(gate_apps_per_s=round(ngates / exact32_s, 1), device_ms=round(exact32_dev_ms, 1),
+                   note="the unfused complex-f32 path (one exact kernel per gate, QuantumStateF32 arithmetic) vs "
+                        "the exact complex-f64 run")
+        err["exact_c32"] = e32
+        del a32, a64, a32x
         log("f32 error: %s" % err)
     out["f32_error_vs_f64"] = err
 
