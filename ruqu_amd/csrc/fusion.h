@@ -57,8 +57,12 @@ struct FPhase {
   int32_t thrt[FUSE_MAX_M - FUSE_R_BITS];  // tile bits holding thread-id bits 0..m-5 (0 past m-5)
   int32_t gthr[FUSE_MAX_M - FUSE_R_BITS];  // global qubits of thread-id bits 0..m-5 (S of thrt)
   int32_t pad[3];
+  // LDS byte offset of register index k's part of the tile index (the
+  // swizzled address of the register bits set in k), filled at upload for
+  // the state's element size
+  uint32_t lreg[1 << FUSE_R_BITS];
 };
-static_assert(sizeof(FPhase) == 128, "phase record: two 64-byte scalar loads");
+static_assert(sizeof(FPhase) == 192, "phase record: three 64-byte scalar loads");
 
 struct FStage {
   int32_t m;            // tile qubits

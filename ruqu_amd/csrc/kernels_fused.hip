@@ -58,7 +58,7 @@ struct Vec2<double> {
 // (thread part | register part) is the XOR of the two parts' addresses, so a
 // phase computes its per-thread part once and each register adds one XOR.
 template <class T>
-__device__ __forceinline__ uint32_t lds_addr(uint32_t t) {
+__host__ __device__ __forceinline__ uint32_t lds_addr(uint32_t t) {
   if constexpr (sizeof(T) == 4) return (t ^ ((t >> 5) & 31u)) * 8u;
   else return (t ^ ((t >> 4) & 15u)) * 16u;
 }
@@ -594,11 +594,11 @@ __device__ __forceinline__ void lds_st(Cx<T>* lds, uint32_t byte, const Cx<T>& x
   *reinterpret_cast<Cx<T>*>(reinterpret_cast<unsigned char*>(lds) + byte) = x;
 }
 
-// A phase record as two 64-byte scalar loads (dword d of the record is
+// A phase record as three 64-byte scalar loads (dword d of the record is
 // word(d)); every field the kernel reads is then in registers at once.
 typedef int32_t i32x16 __attribute__((ext_vector_type(16)));
 struct PhaseWords {
-  i32x16 lo, hi;
+  i32x16 lo, hi, lr;  // lr: the register parts' LDS offsets (FPhase::lreg)
   __device__ __forceinline__ int32_t word(int d) const { return d < 16 ? lo[d] : hi[d - 16]; }
   __device__ __forceinline__ int32_t op_begin() const { return word(offsetof(FPhase, op_begin) / 4); }
   __device__ __forceinline__ int32_t op_end() const { return word(offsetof(FPhase, op_end) / 4); }
@@ -609,10 +609,17 @@ struct PhaseWords {
   __device__ __forceinline__ int32_t greg(int b) const { return word(offsetof(FPhase, greg) / 4 + b); }
   __device__ __forceinline__ int32_t thrt(int j) const { return word(offsetof(FPhase, thrt) / 4 + j); }
   __device__ __forceinline__ int32_t gthr(int j) const { return word(offsetof(FPhase, gthr) / 4 + j); }
+  __device__ __forceinline__ uint32_t lreg(int k) const { return (uint32_t)lr[k]; }
 };
+static_assert(offsetof(FPhase, lreg) == 128, "the LDS offsets form the third 64-byte load");
+// Complex f32 reads the LDS offsets too; the complex-f64 kernel computes
+// them per access instead (16 more scalar registers would make it spill
+// scalar registers throughout its op loop).
+template <class T>
 __device__ __forceinline__ PhaseWords load_phase(const FPhase* p) {
   const i32x16* v = reinterpret_cast<const i32x16*>(p);
-  return PhaseWords{v[0], v[1]};
+  if constexpr (sizeof(T) == 4) return PhaseWords{v[0], v[1], v[2]};
+  else return PhaseWords{v[0], v[1], i32x16{}};
 }
 
 // Branch-free: the block has 2^(m-4) threads, so thread-id bits past m-5 are
@@ -685,6 +692,20 @@ struct Unroll {
     Unroll<K - 1, T>::gather_l(v, lds, mp, lt);
     v[K - 1] = lds_ld(lds, lt ^ lds_addr<T>(treg_of<K - 1>(mp)));
   }
+  // the same with the register parts' offsets from the phase record
+  // (complex f32; see load_phase)
+  static __device__ __forceinline__ void gather_lp(Cx<T> (&v)[FUSE_R], const Cx<T>* lds, const PhaseWords& ph,
+                                                   const Map& mp, uint32_t lt) {
+    Unroll<K - 1, T>::gather_lp(v, lds, ph, mp, lt);
+    if constexpr (sizeof(T) == 4) v[K - 1] = lds_ld(lds, lt ^ ph.lreg(K - 1));
+    else v[K - 1] = lds_ld(lds, lt ^ lds_addr<T>(treg_of<K - 1>(mp)));
+  }
+  static __device__ __forceinline__ void scatter_lp(const Cx<T> (&v)[FUSE_R], Cx<T>* lds, const PhaseWords& ph,
+                                                    const Map& mp, uint32_t lt) {
+    Unroll<K - 1, T>::scatter_lp(v, lds, ph, mp, lt);
+    if constexpr (sizeof(T) == 4) lds_st(lds, lt ^ ph.lreg(K - 1), v[K - 1]);
+    else lds_st(lds, lt ^ lds_addr<T>(treg_of<K - 1>(mp)), v[K - 1]);
+  }
   static __device__ __forceinline__ void scatter_l(const Cx<T> (&v)[FUSE_R], Cx<T>* lds, const Map& mp,
                                                    uint32_t lt) {
     Unroll<K - 1, T>::scatter_l(v, lds, mp, lt);
@@ -697,6 +718,10 @@ struct Unroll<0, T> {
   static __device__ __forceinline__ void gather_g(Cx<T> (&)[FUSE_R], const V*, const Map&) {}
   static __device__ __forceinline__ void scatter_g(const Cx<T> (&)[FUSE_R], V*, const Map&) {}
   static __device__ __forceinline__ void gather_l(Cx<T> (&)[FUSE_R], const Cx<T>*, const Map&, uint32_t) {}
+  static __device__ __forceinline__ void gather_lp(Cx<T> (&)[FUSE_R], const Cx<T>*, const PhaseWords&, const Map&,
+                                                   uint32_t) {}
+  static __device__ __forceinline__ void scatter_lp(const Cx<T> (&)[FUSE_R], Cx<T>*, const PhaseWords&, const Map&,
+                                                    uint32_t) {}
   static __device__ __forceinline__ void scatter_l(const Cx<T> (&)[FUSE_R], Cx<T>*, const Map&, uint32_t) {}
 };
 
@@ -729,7 +754,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   Cx<T> v[FUSE_R];
   const int np = st.nphase;
   for (int p = 0; p < np; ++p) {
-    const PhaseWords ph = load_phase(phases + st.phase0 + p);
+    const PhaseWords ph = load_phase<T>(phases + st.phase0 + p);
     const Map mp = make_map(ph, tid, outer);
     if (p == 0) {
       if (ph.direct()) {
@@ -739,11 +764,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         Unroll<FUSE_R, T>::gather_g(v, reinterpret_cast<const V2*>(a) + nat.gthr, nat);
         Unroll<FUSE_R, T>::scatter_l(v, lds, nat, lane_lds_addr<T>(nat.tthr));
         __syncthreads();
-        Unroll<FUSE_R, T>::gather_l(v, lds, mp, lane_lds_addr<T>(mp.tthr));
+        Unroll<FUSE_R, T>::gather_lp(v, lds, ph, mp, lane_lds_addr<T>(mp.tthr));
         __syncthreads();
       }
     } else {
-      Unroll<FUSE_R, T>::gather_l(v, lds, mp, lane_lds_addr<T>(mp.tthr));
+      Unroll<FUSE_R, T>::gather_lp(v, lds, ph, mp, lane_lds_addr<T>(mp.tthr));
       __syncthreads();
     }
     __builtin_amdgcn_s_setprio(0);
@@ -759,13 +784,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         Unroll<FUSE_R, T>::scatter_g(v, reinterpret_cast<V2*>(a) + mp.gthr, mp);
       } else {
         const Map nat = make_natural_map(st, tid, outer);
-        Unroll<FUSE_R, T>::scatter_l(v, lds, mp, lane_lds_addr<T>(mp.tthr));
+        Unroll<FUSE_R, T>::scatter_lp(v, lds, ph, mp, lane_lds_addr<T>(mp.tthr));
         __syncthreads();
         Unroll<FUSE_R, T>::gather_l(v, lds, nat, lane_lds_addr<T>(nat.tthr));
         Unroll<FUSE_R, T>::scatter_g(v, reinterpret_cast<V2*>(a) + nat.gthr, nat);
       }
     } else {
-      Unroll<FUSE_R, T>::scatter_l(v, lds, mp, lane_lds_addr<T>(mp.tthr));
+      Unroll<FUSE_R, T>::scatter_lp(v, lds, ph, mp, lane_lds_addr<T>(mp.tthr));
       __syncthreads();
     }
   }
@@ -1424,6 +1449,13 @@ int upload_plan_t(State& s, const FusedPlan& plan, FusedProgram* fp) {
   }
   std::vector<unsigned char> host(total, 0);
   std::memcpy(host.data() + o_st, plan.stages.data(), nst * sizeof(FStage));
+  for (FPhase& f : phases)  // the register parts' LDS offsets for this element size
+    for (int k = 0; k < (1 << FUSE_R_BITS); ++k) {
+      uint32_t t = 0;
+      for (int b = 0; b < FUSE_R_BITS; ++b)
+        if ((k >> b) & 1) t |= 1u << f.regt[b];
+      f.lreg[k] = lds_addr<T>(t);
+    }
   std::memcpy(host.data() + o_ph, phases.data(), nph * sizeof(FPhase));
   if constexpr (sizeof(T) == 4) {  // factored 2x2 records carry their f32 coefficients inline
     for (POp& p : pops)
