@@ -342,8 +342,15 @@ int plan_fused(uint32_t nq, const rq_gate* gates, size_t ngates, const FusionCon
 }
 
 int run_fused(State& s, const rq_gate* gates, size_t ngates, const double* draws, size_t ndraws, size_t* cursor,
-              rq_measurement* meas, size_t cap, size_t* produced, FusionStats* st) {
+              rq_measurement* meas, size_t cap, size_t* produced, FusionStats* st, bool zero_first) {
   const double pass_bytes = 2.0 * (double)s.n * (double)s.amp_bytes();
+  // zero_first: the caller wants |0..0> first; the first fused stage makes it
+  // in registers, anything else clears the state first
+  bool zero = zero_first;
+  auto settle = [&]() {
+    if (zero) launch_init_zero(s);
+    zero = false;
+  };
   size_t k = 0;
   while (k < ngates) {
     size_t e = k;
@@ -351,6 +358,7 @@ int run_fused(State& s, const rq_gate* gates, size_t ngates, const double* draws
     if (e > k) {
       if (s.nq < (uint32_t)FUSE_R_BITS) {
         // Tiny states: the exact per-gate kernels.
+        settle();
         for (size_t j = k; j < e; ++j) {
           int has = 0;
           rq_measurement mm;
@@ -364,8 +372,9 @@ int run_fused(State& s, const rq_gate* gates, size_t ngates, const double* draws
         }
       } else {
         uint64_t unitary = 0, stages = 0;
-        int rc = run_fused_segment(s, gates + k, e - k, &unitary, &stages);
+        int rc = run_fused_segment(s, gates + k, e - k, &unitary, &stages, &zero);
         if (rc) return rc;
+        settle();  // a segment without stages leaves the state as it was
         st->unitary += unitary;
         st->passes += stages;
         st->hbm_bytes += pass_bytes * (double)stages;
@@ -373,6 +382,7 @@ int run_fused(State& s, const rq_gate* gates, size_t ngates, const double* draws
       }
     }
     if (e < ngates) {
+      settle();
       const rq_gate& g = gates[e];
       if (*cursor >= ndraws) return fail(RQ_ERR_CIRCUIT, "measurement needs a uniform draw but the draw list is exhausted");
       double draw = draws[(*cursor)++];
@@ -392,6 +402,7 @@ int run_fused(State& s, const rq_gate* gates, size_t ngates, const double* draws
     }
     k = e + 1;
   }
+  settle();
   st->fused = st->unitary > st->passes ? st->unitary - st->passes : 0;
   return RQ_OK;
 }

@@ -92,7 +92,8 @@ struct FusionConfig {
 // Executes gates[0..ngates) with the fused schedule; Measure/Reset gates run
 // exactly (via measure/reset on the state) between fused segments.
 int run_fused(State& s, const rq_gate* gates, size_t ngates, const double* draws, size_t ndraws, size_t* cursor,
-              rq_measurement* meas, size_t cap, size_t* produced, FusionStats* st);
+              rq_measurement* meas, size_t cap, size_t* produced, FusionStats* st,
+              bool zero_first = false);
 // Same contract with dense 5-qubit blocks on the matrix cores (dense.cpp).
 int run_dense(State& s, const rq_gate* gates, size_t ngates, const double* draws, size_t ndraws, size_t* cursor,
               rq_measurement* meas, size_t cap, size_t* produced, FusionStats* st);
@@ -111,7 +112,12 @@ int plan_fused(uint32_t nq, const rq_gate* gates, size_t ngates, const FusionCon
 int launch_fused_plan(State& s, const FusedPlan& plan);
 // Plan (or reuse the state's cached program for the same gates and knobs),
 // upload and launch one segment without Measure/Reset gates.
-int run_fused_segment(State& s, const rq_gate* gates, size_t ngates, uint64_t* unitary, uint64_t* stages);
+// With *zero_in set, the state is taken as |0..0> without reading it (its
+// memory may hold anything): on a complex-f32 state the first stage makes the
+// amplitudes, on a complex-f64 state a clearing pass runs first; *zero_in is
+// cleared once a stage ran (a segment without stages leaves it set).
+int run_fused_segment(State& s, const rq_gate* gates, size_t ngates, uint64_t* unitary, uint64_t* stages,
+                      bool* zero_in = nullptr);
 
 // Host execution of the program the kernel receives for a plan (packed op
 // records, folded gates, matrix-core phases) with the kernel's data mapping

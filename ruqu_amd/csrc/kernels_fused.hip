@@ -680,6 +680,11 @@ struct Unroll {
     const V x = at[greg_of<K - 1>(mp)];
     v[K - 1] = Cx<T>{x.x, x.y};
   }
+  // |0..0>: 1 at global index 0, 0 elsewhere
+  static __device__ __forceinline__ void zero_state(Cx<T> (&v)[FUSE_R], const Map& mp) {
+    Unroll<K - 1, T>::zero_state(v, mp);
+    v[K - 1] = Cx<T>{(mp.gthr | greg_of<K - 1>(mp)) == 0 ? T(1) : T(0), T(0)};
+  }
   static __device__ __forceinline__ void scatter_g(const Cx<T> (&v)[FUSE_R], V* at, const Map& mp) {
     Unroll<K - 1, T>::scatter_g(v, at, mp);
     V x;
@@ -716,6 +721,7 @@ template <class T>
 struct Unroll<0, T> {
   using V = typename Vec2<T>::type;
   static __device__ __forceinline__ void gather_g(Cx<T> (&)[FUSE_R], const V*, const Map&) {}
+  static __device__ __forceinline__ void zero_state(Cx<T> (&)[FUSE_R], const Map&) {}
   static __device__ __forceinline__ void scatter_g(const Cx<T> (&)[FUSE_R], V*, const Map&) {}
   static __device__ __forceinline__ void gather_l(Cx<T> (&)[FUSE_R], const Cx<T>*, const Map&, uint32_t) {}
   static __device__ __forceinline__ void gather_lp(Cx<T> (&)[FUSE_R], const Cx<T>*, const PhaseWords&, const Map&,
@@ -728,7 +734,7 @@ struct Unroll<0, T> {
 template <class T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_fused_stage(T* __restrict__ a, const FStage* __restrict__ stp,
                                                       const FPhase* __restrict__ phases, const POp* __restrict__ ops,
-                                                      const Cx<T>* __restrict__ pool) {
+                                                      const Cx<T>* __restrict__ pool, int zero_in) {
   // A static array (the kernel's only LDS) sits at LDS address 0, so the
   // compiler folds its base into the addresses.
   __shared__ Cx<T> lds[1 << FUSE_MAX_M];
@@ -756,7 +762,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   for (int p = 0; p < np; ++p) {
     const PhaseWords ph = load_phase<T>(phases + st.phase0 + p);
     const Map mp = make_map(ph, tid, outer);
-    if (p == 0) {
+    if (sizeof(T) == 4 && p == 0 && zero_in) {
+      // the program's first stage on |0..0>: the amplitudes are made here
+      // (no clearing pass before, no read of the state); complex f32 only
+      // (in the complex-f64 kernel the branch costs scalar-register spills)
+      const Map& m0 = mp;
+      if (ph.direct()) {
+        Unroll<FUSE_R, T>::zero_state(v, m0);
+      } else {
+        const Map nat = make_natural_map(st, tid, outer);
+        Unroll<FUSE_R, T>::zero_state(v, nat);
+        Unroll<FUSE_R, T>::scatter_l(v, lds, nat, lane_lds_addr<T>(nat.tthr));
+        __syncthreads();
+        Unroll<FUSE_R, T>::gather_lp(v, lds, ph, mp, lane_lds_addr<T>(mp.tthr));
+        __syncthreads();
+      }
+    } else if (p == 0) {
       if (ph.direct()) {
         Unroll<FUSE_R, T>::gather_g(v, reinterpret_cast<const V2*>(a) + mp.gthr, mp);
       } else {
@@ -1486,7 +1507,7 @@ int upload_plan_t(State& s, const FusedPlan& plan, FusedProgram* fp) {
 
 // One launch per stage of the program in s.prog (asynchronous on s.stream).
 template <class T>
-int launch_program_t(State& s, const FusedProgram& fp) {
+int launch_program_t(State& s, const FusedProgram& fp, bool zero_in) {
   const unsigned char* base = reinterpret_cast<const unsigned char*>(s.prog);
   for (size_t i = 0; i < fp.stages.size(); ++i) {
     const FStage& st = fp.stages[i];
@@ -1495,7 +1516,7 @@ int launch_program_t(State& s, const FusedProgram& fp) {
     hipLaunchKernelGGL(k_fused_stage<T>, dim3((unsigned)tiles), dim3(threads), 0, s.stream, (T*)s.amps,
                        reinterpret_cast<const FStage*>(base + fp.o_st + i * sizeof(FStage)),
                        reinterpret_cast<const FPhase*>(base + fp.o_ph), reinterpret_cast<const POp*>(base + fp.o_op),
-                       reinterpret_cast<const Cx<T>*>(base + fp.o_pool));
+                       reinterpret_cast<const Cx<T>*>(base + fp.o_pool), (zero_in && i == 0) ? 1 : 0);
   }
   RQ_HIP(hipGetLastError());
   return RQ_OK;
@@ -1730,12 +1751,13 @@ int launch_fused_plan(State& s, const FusedPlan& plan) {
   FusedProgram fp;
   int rc = s.prec == RQ_C64 ? upload_plan_t<double>(s, plan, &fp) : upload_plan_t<float>(s, plan, &fp);
   if (rc) return rc;
-  return s.prec == RQ_C64 ? launch_program_t<double>(s, fp) : launch_program_t<float>(s, fp);
+  return s.prec == RQ_C64 ? launch_program_t<double>(s, fp, false) : launch_program_t<float>(s, fp, false);
 }
 
 // Plans (unless the state's cached program was planned from the same gates,
 // size, precision and knobs), uploads, and launches one fused segment.
-int run_fused_segment(State& s, const rq_gate* gates, size_t ngates, uint64_t* unitary, uint64_t* stages) {
+int run_fused_segment(State& s, const rq_gate* gates, size_t ngates, uint64_t* unitary, uint64_t* stages,
+                      bool* zero_in) {
   const FusionConfig cfg = fusion_config();
   FusedProgram* fp = s.fused;
   const bool hit = fp && fp->valid && fp->nq == s.nq && fp->prec == s.prec && same_config(fp->cfg, cfg) &&
@@ -1755,8 +1777,12 @@ int run_fused_segment(State& s, const rq_gate* gates, size_t ngates, uint64_t* u
     fp->cfg = cfg;
     fp->valid = true;
   }
-  int rc = s.prec == RQ_C64 ? launch_program_t<double>(s, *fp) : launch_program_t<float>(s, *fp);
+  const bool want = zero_in && *zero_in && !fp->stages.empty();
+  const bool z = want && s.prec == RQ_C32;
+  if (want && !z) launch_init_zero(s);  // complex f64: cleared by a pass of its own
+  int rc = s.prec == RQ_C64 ? launch_program_t<double>(s, *fp, z) : launch_program_t<float>(s, *fp, z);
   if (rc) return rc;
+  if (want) *zero_in = false;
   *unitary = fp->unitary;
   *stages = fp->stages.size();
   return RQ_OK;

@@ -261,8 +261,11 @@ int run_gates(State& s, const rq_gate* gates, size_t ngates, const double* draws
     RQ_HIP(hipEventCreate(&e0));
     RQ_HIP(hipEventCreate(&e1));
   }
+  // A fused run from |0..0> lets its first stage make the amplitudes instead
+  // of clearing the state first (run_fused clears it when no stage runs).
+  const bool zero_first = reset && fusion == 1;
   if (reset) {
-    launch_init_zero(s);
+    if (!zero_first) launch_init_zero(s);
     s.record.clear();
     s.gate_count = 0;
   }
@@ -274,7 +277,7 @@ int run_gates(State& s, const rq_gate* gates, size_t ngates, const double* draws
   if (fusion) {
     FusionStats st{};
     rc = fusion == 2 ? run_dense(s, gates, ngates, draws, ndraws, &cursor, meas, cap, &produced, &st)
-                     : run_fused(s, gates, ngates, draws, ndraws, &cursor, meas, cap, &produced, &st);
+                     : run_fused(s, gates, ngates, draws, ndraws, &cursor, meas, cap, &produced, &st, zero_first);
     unitary = st.unitary;
     passes = st.passes;
     fused = st.fused;
