@@ -540,8 +540,29 @@ constexpr int MF_FLOATS = 1024;
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-__device__ __forceinline__ void apply_mf(Cx<float> (&v)[FUSE_R], const float* __restrict__ w, uint32_t sel,
-                                         uint32_t tid) {
+// The A operand of a matrix-core phase's chain 0 (16 floats per lane, all
+// four loads issued before the first product).  Chain 1 reuses it unless the
+// unitary depends on the chain (cb != 0, wave-uniform); then chain 1's operand
+// is loaded while chain 0 runs.  The chains run one after the other, so at
+// most 32 operand registers are live (the two chains interleaved with both
+// operands held 32 more, and the compiler then issued each load just before
+// its products and waited for it).  Loading the operand before the phase's
+// LDS gather made the kernel spill.
+struct MfA {
+  float a0[16];
+  const float4* w1;
+  uint32_t cb;
+};
+
+__device__ __forceinline__ void mf_load(float (&a)[16], const float4* __restrict__ w) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float4 x = w[q * 64];
+    a[4 * q] = x.x; a[4 * q + 1] = x.y; a[4 * q + 2] = x.z; a[4 * q + 3] = x.w;
+  }
+}
+
+__device__ __forceinline__ void mf_fetch(MfA& m, const float* __restrict__ w, uint32_t sel, uint32_t tid) {
   uint32_t vb = 0, cb = 0;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -552,27 +573,33 @@ __device__ __forceinline__ void apply_mf(Cx<float> (&v)[FUSE_R], const float* __
   }
   vb = __builtin_amdgcn_readfirstlane(vb);
   const uint32_t lane = tid & 63u;
-  const float4* __restrict__ w0 = reinterpret_cast<const float4*>(w + vb * MF_FLOATS) + lane;
-  const float4* __restrict__ w1 = reinterpret_cast<const float4*>(w + (vb | cb) * MF_FLOATS) + lane;
-  float a0[16], a1[16];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const float4 x0 = w0[q * 64], x1 = w1[q * 64];
-    a0[4 * q] = x0.x; a0[4 * q + 1] = x0.y; a0[4 * q + 2] = x0.z; a0[4 * q + 3] = x0.w;
-    a1[4 * q] = x1.x; a1[4 * q + 1] = x1.y; a1[4 * q + 2] = x1.z; a1[4 * q + 3] = x1.w;
-  }
-  f32x16 d0 = {}, d1 = {};
+  mf_load(m.a0, reinterpret_cast<const float4*>(w + vb * MF_FLOATS) + lane);
+  m.cb = cb;
+  m.w1 = reinterpret_cast<const float4*>(w + (vb | cb) * MF_FLOATS) + lane;
+}
+
+// One chain: D = A X over the 8 amplitudes v[8c .. 8c + 7] of this lane.
+template <int C>
+__device__ __forceinline__ void mf_chain(Cx<float> (&v)[FUSE_R], const float (&a)[16]) {
+  f32x16 d = {};
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
-    const float x0 = (s & 1) ? v[s >> 1].im : v[s >> 1].re;
-    const float x1 = (s & 1) ? v[8 + (s >> 1)].im : v[8 + (s >> 1)].re;
-    d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[s], x0, d0, 0, 0, 0);
-    d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[s], x1, d1, 0, 0, 0);
+    const float x = (s & 1) ? v[8 * C + (s >> 1)].im : v[8 * C + (s >> 1)].re;
+    d = __builtin_amdgcn_mfma_f32_32x32x2f32(a[s], x, d, 0, 0, 0);
   }
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    v[j] = Cx<float>{d0[2 * j], d0[2 * j + 1]};
-    v[8 + j] = Cx<float>{d1[2 * j], d1[2 * j + 1]};
+  for (int j = 0; j < 8; ++j) v[8 * C + j] = Cx<float>{d[2 * j], d[2 * j + 1]};
+}
+
+__device__ __forceinline__ void apply_mf(Cx<float> (&v)[FUSE_R], const MfA& m) {
+  if (m.cb) {
+    float a1[16];
+    mf_load(a1, m.w1);
+    mf_chain<0>(v, m.a0);
+    mf_chain<1>(v, a1);
+  } else {
+    mf_chain<0>(v, m.a0);
+    mf_chain<1>(v, m.a0);
   }
 }
 
@@ -794,8 +821,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     }
     __builtin_amdgcn_s_setprio(0);
     if constexpr (sizeof(T) == 4) {
-      if (ph.mf_off() >= 0) apply_mf(v, reinterpret_cast<const float*>(pool + ph.mf_off()), ph.mf_sel(), tid);
-      else apply_ops(v, ops, ph.op_begin(), ph.op_end(), pool, tid);
+      if (ph.mf_off() >= 0) {
+        MfA mfa;
+        mf_fetch(mfa, reinterpret_cast<const float*>(pool + ph.mf_off()), ph.mf_sel(), tid);
+        apply_mf(v, mfa);
+      } else {
+        apply_ops(v, ops, ph.op_begin(), ph.op_end(), pool, tid);
+      }
     } else {
       apply_ops(v, ops, ph.op_begin(), ph.op_end(), pool, tid);
     }
