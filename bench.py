@@ -179,7 +179,7 @@ def headline(args, L):
         "traffic": (measured_value(os.path.join(ROOT, "profiles", "fused_stage_30q_c32_hbm_traffic.md"),
                                    "traffic_bytes_per_launch") if (nq, prec, args.depth) == (30, "c32", 40) else None),
         "kernel": "k_fused_stage", "bytes_per_launch": stage_bytes, "mean_launch_ms": round(mean_stage_ms, 4),
-        "limiter": "VALU issue and the scalar-load and LDS latency 5 waves per SIMD leave exposed (roofline_valu)"}
+        "limiter": "VALU issue and the scalar-load and LDS latency 4 waves per SIMD leave exposed (roofline_valu)"}
 
     # FP32 vector work of the packed program (the matrix-core phases apart)
     cnt = np.zeros(9, dtype=np.uint64)
