@@ -778,12 +778,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
 #pragma unroll
     for (int j = 0; j < 32; ++j) outer |= ((tile >> j) & 1ull) << ((j < 16 ? o0[j] : o1[j - 16]) & 63);
   }
-  // Wave priority: high while moving data (HBM loads and stores, LDS
-  // transposes), low while applying ops.  The tiles of a stage all do the
-  // same work, so their waves drift in step through the memory and the
-  // arithmetic sections; letting the data movement of a wave that starts or
-  // ends a tile go first keeps HBM busy while the other waves compute.
-  __builtin_amdgcn_s_setprio(3);
+  // All waves keep the default priority: raising it for the data movement
+  // (HBM loads and stores, LDS transposes) once paid, and with the current
+  // op records it costs 0.3-0.8 % (raising it for the ops instead, 3 %).
   Cx<T> v[FUSE_R];
   const int np = st.nphase;
   for (int p = 0; p < np; ++p) {
@@ -819,7 +816,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
       Unroll<FUSE_R, T>::gather_lp(v, lds, ph, mp, lane_lds_addr<T>(mp.tthr));
       __syncthreads();
     }
-    __builtin_amdgcn_s_setprio(0);
     if constexpr (sizeof(T) == 4) {
       if (ph.mf_off() >= 0) {
         MfA mfa;
@@ -831,7 +827,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     } else {
       apply_ops(v, ops, ph.op_begin(), ph.op_end(), pool, tid);
     }
-    __builtin_amdgcn_s_setprio(3);
     if (p == np - 1) {
       if (ph.direct()) {
         Unroll<FUSE_R, T>::scatter_g(v, reinterpret_cast<V2*>(a) + mp.gthr, mp);
