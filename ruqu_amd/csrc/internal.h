@@ -48,6 +48,8 @@ struct State {
   double* host_red = nullptr;  // pinned host mirror for small results
   std::vector<rq_measurement> record;
   uint64_t gate_count = 0;   // for precision_error_bound
+  void* samp = nullptr;      // shot sampler's tree, draws and bits, kept when at most SAMPLER_KEEP_BYTES
+  size_t samp_cap = 0;
   void* prog = nullptr;      // device copy of the fused / dense program
   size_t prog_cap = 0;
   FusedProgram* fused = nullptr;  // what `prog` holds when it is a reusable fused program

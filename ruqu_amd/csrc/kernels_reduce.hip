@@ -493,20 +493,38 @@ int launch_probabilities(State& s, double* d_out) {
   return RQ_OK;
 }
 
+// The sampler's device memory (sum tree, draws, bits, tie counter) is one
+// block.  Blocks up to 4 GiB stay with the state for its next sampling
+// (repeated sampling of a 26-qubit state then allocates and frees nothing);
+// larger ones are released after the call.
+constexpr size_t SAMPLER_KEEP_BYTES = size_t(4) << 30;
+
 int sample_shots(State& s, const double* draws, size_t shots, uint8_t* bits, uint64_t* near_ties) {
   if (shots == 0) {
     if (near_ties) *near_ties = 0;
     return RQ_OK;
   }
-  double* tree = nullptr;
-  double* d_draws = nullptr;
-  uint8_t* d_bits = nullptr;
-  unsigned long long* d_ties = nullptr;
   size_t nd = shots * s.nq;
-  hipError_t e = hipMalloc((void**)&tree, 2 * s.n * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc((void**)&d_draws, (nd ? nd : 1) * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc((void**)&d_bits, nd ? nd : 1);
-  if (e == hipSuccess) e = hipMalloc((void**)&d_ties, sizeof(unsigned long long));
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t o_draws = al(2 * s.n * sizeof(double)), o_bits = o_draws + al(nd * sizeof(double)),
+               o_ties = o_bits + al(nd), need = o_ties + al(sizeof(unsigned long long));
+  hipError_t e = hipSuccess;
+  if (s.samp_cap < need) {
+    if (s.samp) {
+      e = hipStreamSynchronize(s.stream);
+      if (e == hipSuccess) e = hipFree(s.samp);
+      s.samp = nullptr;
+      s.samp_cap = 0;
+    }
+    if (e == hipSuccess) e = hipMalloc(&s.samp, need);
+    if (e == hipSuccess) s.samp_cap = need;
+    else s.samp = nullptr;
+  }
+  unsigned char* base = reinterpret_cast<unsigned char*>(s.samp);
+  double* tree = reinterpret_cast<double*>(base);
+  double* d_draws = reinterpret_cast<double*>(base + o_draws);
+  uint8_t* d_bits = base + o_bits;
+  unsigned long long* d_ties = reinterpret_cast<unsigned long long*>(base + o_ties);
   int rc = RQ_OK;
   if (e != hipSuccess) {
     rc = fail(RQ_ERR_ALLOC, "sample_shots: device allocation failed");
@@ -536,10 +554,11 @@ int sample_shots(State& s, const double* draws, size_t shots, uint8_t* bits, uin
     else if (near_ties) *near_ties = ties;
   }
   (void)hipStreamSynchronize(s.stream);
-  if (tree) (void)hipFree(tree);
-  if (d_draws) (void)hipFree(d_draws);
-  if (d_bits) (void)hipFree(d_bits);
-  if (d_ties) (void)hipFree(d_ties);
+  if (s.samp && (rc != RQ_OK || s.samp_cap > SAMPLER_KEEP_BYTES)) {
+    (void)hipFree(s.samp);
+    s.samp = nullptr;
+    s.samp_cap = 0;
+  }
   return rc;
 }
 

@@ -460,6 +460,7 @@ void rq_state_destroy(rq_state* h) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   if (s->amps) (void)hipFree(s->amps);
   if (s->red) (void)hipFree(s->red);
+  if (s->samp) (void)hipFree(s->samp);
   if (s->prog) (void)hipFree(s->prog);
   release_fused_program(*s);
   if (s->host_red) (void)hipHostFree(s->host_red);
